@@ -1,0 +1,192 @@
+/*
+ * qsv.h — C ABI of libqsv, the MI355X state-vector engine behind QClojure's
+ * backend protocol.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes.
+ * Every call returns QSV_OK (0) or a negative QSV_E* code; the message of the
+ * last failure on the calling thread is returned by qsv_last_error().
+ *
+ * Conventions shared with the reference (lsolbach/qclojure):
+ *   - Qubit q is the MOST significant index bit: basis index
+ *     i = sum_q b_q * 2^(n-1-q)            (src/org/soulspace/qclojure/domain/state.clj:114-136)
+ *   - Amplitudes cross the ABI as interleaved FP64 pairs (re, im), the
+ *     layout of the reference's fastmath Vec2 state vectors
+ *     (src/org/soulspace/qclojure/domain/state.clj:9-35).
+ *   - Operation kinds and operand order follow the dispatcher
+ *     apply-gate-to-state (src/org/soulspace/qclojure/domain/circuit.clj:933-1072).
+ *
+ * Each entry point cites the reference function it replaces.
+ */
+#ifndef QSV_H
+#define QSV_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define QSV_OK              0
+#define QSV_E_INVALID      -1   /* bad argument (qubit out of range, null pointer ...) */
+#define QSV_E_UNSUPPORTED  -2   /* operation the reference also rejects              */
+#define QSV_E_NOMEM        -3   /* device allocation / HBM budget exhausted            */
+#define QSV_E_DEVICE       -4   /* HIP runtime error                                   */
+#define QSV_E_STATE        -5   /* wrong object state (not initialised, job not done)  */
+#define QSV_E_NOTFOUND     -6   /* unknown job id                                       */
+
+/* ---- precision ----------------------------------------------------------- */
+#define QSV_C128 0   /* complex128: FP64 re/im, 16 B per amplitude */
+#define QSV_C64  1   /* complex64 : FP32 re/im,  8 B per amplitude; FP64 reductions */
+
+/* ---- operation kinds (src/org/soulspace/qclojure/domain/circuit.clj:964-1072) ----
+ * Operand slots q[] per kind:
+ *   1-qubit gates ..................... q[0]=target
+ *   CNOT/CZ/CY/CRX/CRY/CRZ/RYDBERG_* .. q[0]=control, q[1]=target
+ *   SWAP/ISWAP ........................ q[0]=qubit1,  q[1]=qubit2
+ *   TOFFOLI ........................... q[0]=control1, q[1]=control2, q[2]=target
+ *   FREDKIN ........................... q[0]=control, q[1]=target1, q[2]=target2
+ *   RYDBERG_BLOCKADE, MEASURE ......... mask = set of qubits (bit q = qubit q)
+ *   U ................................. q[0]=target, m[] = row-major 2x2 (re,im)
+ *   CU ................................ q[0]=control, q[1]=target, m[] as U
+ */
+enum qsv_op_kind {
+  QSV_OP_X = 1, QSV_OP_Y, QSV_OP_Z, QSV_OP_H,
+  QSV_OP_S, QSV_OP_SDG, QSV_OP_T, QSV_OP_TDG,
+  QSV_OP_RX, QSV_OP_RY, QSV_OP_RZ, QSV_OP_PHASE,
+  QSV_OP_CNOT, QSV_OP_CZ, QSV_OP_CY, QSV_OP_CRX, QSV_OP_CRY, QSV_OP_CRZ,
+  QSV_OP_SWAP, QSV_OP_ISWAP, QSV_OP_TOFFOLI, QSV_OP_FREDKIN,
+  QSV_OP_RYDBERG_CZ, QSV_OP_RYDBERG_CPHASE, QSV_OP_RYDBERG_BLOCKADE,
+  QSV_OP_GLOBAL_RX, QSV_OP_GLOBAL_RY, QSV_OP_GLOBAL_RZ, QSV_OP_GLOBAL_H,
+  QSV_OP_GLOBAL_X, QSV_OP_GLOBAL_Y, QSV_OP_GLOBAL_Z,
+  QSV_OP_MEASURE,
+  QSV_OP_U, QSV_OP_CU,
+  QSV_OP_KIND_END
+};
+
+/* One decoded circuit operation: a fixed-size POD (96 bytes). */
+typedef struct qsv_op {
+  int32_t  kind;     /* enum qsv_op_kind                           */
+  int32_t  q[3];     /* qubit operands, reference (MSB) numbering  */
+  uint64_t mask;     /* qubit set for BLOCKADE / MEASURE           */
+  double   angle;    /* rotation / phase angle in radians          */
+  double   m[8];     /* explicit 2x2 matrix for U / CU             */
+} qsv_op;
+
+/* apply flags */
+#define QSV_FLAG_MSB_SWAP  0x1u  /* SWAP/ISWAP on MSB bit positions. Default is the
+                                    reference's LSB positions (gate.clj:768-778,823-833). */
+#define QSV_FLAG_FUSE      0x2u  /* batch runs of gates into LDS-tiled passes */
+
+/* ---- library / device --------------------------------------------------- */
+typedef struct qsv_device_info {
+  char     name[64];
+  char     arch[32];        /* e.g. "gfx950"                         */
+  int32_t  compute_units;
+  int32_t  wavefront_size;
+  uint64_t hbm_bytes;       /* total device memory                    */
+  uint64_t hbm_free_bytes;
+} qsv_device_info;
+
+/* Replaces the implicit JVM heap of LocalQuantumSimulator
+ * (src/org/soulspace/qclojure/adapter/backend/ideal_simulator.clj:181-195). */
+int         qsv_init(int device);
+int         qsv_shutdown(void);
+int         qsv_device_query(qsv_device_info* out);
+const char* qsv_last_error(void);
+const char* qsv_version(void);
+
+/* ---- state vectors ------------------------------------------------------ */
+typedef struct qsv_state qsv_state;
+
+/* zero-state / computational-basis-state (state.clj:251-284, 484-519). */
+int qsv_state_create(int num_qubits, int precision, qsv_state** out);
+int qsv_state_destroy(qsv_state* s);
+int qsv_state_num_qubits(const qsv_state* s);
+int qsv_state_precision(const qsv_state* s);
+int qsv_state_init_basis(qsv_state* s, uint64_t index);
+/* :initial-state option (ideal_simulator.clj:85-87): interleaved FP64 (re,im). */
+int qsv_state_upload(qsv_state* s, const double* amps, uint64_t offset, uint64_t count);
+/* :final-state materialisation (circuit.clj:1783): interleaved FP64 (re,im). */
+int qsv_state_download(const qsv_state* s, double* amps, uint64_t offset, uint64_t count);
+int qsv_state_copy(qsv_state* dst, const qsv_state* src);
+int qsv_state_sync(qsv_state* s);
+
+/* ---- gate application --------------------------------------------------- */
+/* reduce apply-operation-to-state over the ops (circuit.clj:1782, 1094-1111);
+ * seed drives QSV_OP_MEASURE collapses (state.clj:915-1014). */
+int qsv_apply_program(qsv_state* s, const qsv_op* ops, uint64_t n_ops,
+                      uint32_t flags, uint64_t seed);
+
+/* ---- result extraction -------------------------------------------------- */
+/* cla/norm2 squared, FP64 accumulation (complex_linear_algebra.clj:435-446). */
+int qsv_norm2(const qsv_state* s, double* out);
+/* normalize-state (state.clj:524-551). */
+int qsv_normalize(qsv_state* s);
+/* measurement-probabilities / probabilities (state.clj:658-682, 1071-1092), FP64. */
+int qsv_probabilities(const qsv_state* s, uint64_t offset, uint64_t count, double* out);
+/* measure-state repeated `shots` times (result.clj:224, state.clj:869-913):
+ * outcome k uses uniform u_k = qsv_uniform(seed, k). */
+int qsv_sample(const qsv_state* s, uint64_t shots, uint64_t seed, uint64_t* outcomes);
+/* measure-specific-qubits (state.clj:915-1014): collapse `mask` using uniform u.
+ * outcome bit j = value of the j-th measured qubit in ascending qubit order;
+ * probs (optional, 2^popcount(mask) doubles) receives every outcome probability. */
+int qsv_measure_qubits(qsv_state* s, uint64_t mask, double u,
+                       uint64_t* outcome, double* probs);
+/* pauli-string-expectation per term (hamiltonian.clj:82-94, observables.clj:216-251):
+ * `strings` holds n_terms strings of num_qubits chars from "IXYZ", packed back to back;
+ * out[k] = Re <psi|P_k|psi> in FP64. The caller sums c_k*out[k] in term order. */
+int qsv_expect_pauli(const qsv_state* s, const char* strings, uint64_t n_terms, double* out);
+/* expectation-value of a 1-qubit observable embedded as I..O..I
+ * (result.clj:254-288, observables.clj:234-251): out = {re, im} of <psi|O_q|psi>. */
+int qsv_expect_1q(const qsv_state* s, int qubit, const double m[8], double out[2]);
+/* inner product <a|b> with FP64 accumulation (state.clj:1164-1185 takes |.|). */
+int qsv_inner(const qsv_state* a, const qsv_state* b, double out[2]);
+
+/* Counter-based uniform in [0,1) used by qsv_sample / MEASURE: splitmix64 of
+ * (seed + (k+1) * 0x9E3779B97F4A7C15), top 53 bits. Exposed so callers can
+ * reproduce shot streams. */
+double qsv_uniform(uint64_t seed, uint64_t k);
+
+/* ---- batched circuits (BatchJobBackend, application/backend.clj:136-165) --
+ * Many independent circuits, each on its own state, applied in one persistent
+ * launch. Instance i runs ops[op_offsets[i] .. op_offsets[i+1]) on a
+ * num_qubits[i]-qubit |0..0> state; n_terms_i Pauli strings (Z-only) are
+ * evaluated on the final state into energies[i] = sum_k coeff_k <P_k> (FP64,
+ * term order). */
+typedef struct qsv_batch_desc {
+  uint64_t        n_instances;
+  const int32_t*  num_qubits;     /* [n_instances]                                   */
+  const uint64_t* op_offsets;     /* [n_instances+1] into ops                        */
+  const qsv_op*   ops;
+  const uint64_t* term_offsets;   /* [n_instances+1] into term_zmask / term_coeff    */
+  const uint64_t* term_zmask;     /* Z-string as a qubit mask (bit q = Z on qubit q) */
+  const double*   term_coeff;
+  int32_t         precision;      /* QSV_C64 or QSV_C128                             */
+} qsv_batch_desc;
+
+int qsv_batch_expect_z(const qsv_batch_desc* desc, double* energies);
+
+/* ---- per-kernel timing (measurement support) ----------------------------
+ * When enabled, qsv_apply_program brackets every kernel it launches with HIP
+ * events on the state's own stream and accumulates (count, total ms,
+ * algorithmic bytes) per kernel class. */
+#define QSV_KCLASS_PAIR    0   /* dense 2x2 on a pair (optionally controlled) */
+#define QSV_KCLASS_DIAG    1   /* diagonal phase                              */
+#define QSV_KCLASS_PERM    2   /* swap-type permutation                       */
+#define QSV_KCLASS_BLOCK   3   /* fused LDS-tiled pass                        */
+#define QSV_KCLASS_OTHER   4
+#define QSV_KCLASS_COUNT   5
+typedef struct qsv_kernel_stat {
+  uint64_t launches;
+  double   total_ms;
+  double   bytes;          /* algorithmic bytes over all launches */
+} qsv_kernel_stat;
+int qsv_timing_enable(qsv_state* s, int enable);
+int qsv_timing_read(qsv_state* s, qsv_kernel_stat out[QSV_KCLASS_COUNT], int reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QSV_H */

@@ -1,0 +1,160 @@
+"""ctypes binding of libqsv's C ABI (include/qsv.h).
+
+The shared library is built in-tree (``qclojure_amd/libqsv.so``). Loading it
+never falls back to anything: if the library is missing the import of the
+engine raises, naming the build step.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libqsv.so")
+
+# ---- constants mirrored from include/qsv.h --------------------------------
+QSV_OK = 0
+QSV_E_INVALID = -1
+QSV_E_UNSUPPORTED = -2
+QSV_E_NOMEM = -3
+QSV_E_DEVICE = -4
+QSV_E_STATE = -5
+QSV_E_NOTFOUND = -6
+
+QSV_C128 = 0
+QSV_C64 = 1
+
+QSV_FLAG_MSB_SWAP = 0x1
+QSV_FLAG_FUSE = 0x2
+
+_KIND_NAMES = [
+    "X", "Y", "Z", "H", "S", "SDG", "T", "TDG", "RX", "RY", "RZ", "PHASE",
+    "CNOT", "CZ", "CY", "CRX", "CRY", "CRZ",
+    "SWAP", "ISWAP", "TOFFOLI", "FREDKIN",
+    "RYDBERG_CZ", "RYDBERG_CPHASE", "RYDBERG_BLOCKADE",
+    "GLOBAL_RX", "GLOBAL_RY", "GLOBAL_RZ", "GLOBAL_H",
+    "GLOBAL_X", "GLOBAL_Y", "GLOBAL_Z",
+    "MEASURE", "U", "CU",
+]
+OP = {name: i + 1 for i, name in enumerate(_KIND_NAMES)}
+OP_NAME = {v: k for k, v in OP.items()}
+
+KCLASS_NAMES = ["pair", "diag", "perm", "block", "other"]
+
+# qsv_op: int32 kind; int32 q[3]; uint64 mask; double angle; double m[8]  (96 bytes)
+OP_DTYPE = np.dtype([
+    ("kind", "<i4"), ("q", "<i4", (3,)), ("mask", "<u8"),
+    ("angle", "<f8"), ("m", "<f8", (8,)),
+], align=True)
+assert OP_DTYPE.itemsize == 96
+
+
+class QsvOp(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("q", ctypes.c_int32 * 3),
+                ("mask", ctypes.c_uint64), ("angle", ctypes.c_double),
+                ("m", ctypes.c_double * 8)]
+
+
+class QsvDeviceInfo(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 64), ("arch", ctypes.c_char * 32),
+                ("compute_units", ctypes.c_int32), ("wavefront_size", ctypes.c_int32),
+                ("hbm_bytes", ctypes.c_uint64), ("hbm_free_bytes", ctypes.c_uint64)]
+
+
+class QsvKernelStat(ctypes.Structure):
+    _fields_ = [("launches", ctypes.c_uint64), ("total_ms", ctypes.c_double),
+                ("bytes", ctypes.c_double)]
+
+
+class QsvBatchDesc(ctypes.Structure):
+    _fields_ = [("n_instances", ctypes.c_uint64),
+                ("num_qubits", ctypes.c_void_p),
+                ("op_offsets", ctypes.c_void_p),
+                ("ops", ctypes.c_void_p),
+                ("term_offsets", ctypes.c_void_p),
+                ("term_zmask", ctypes.c_void_p),
+                ("term_coeff", ctypes.c_void_p),
+                ("precision", ctypes.c_int32)]
+
+
+# Every symbol include/qsv.h declares, with (restype, argtypes).
+_vp, _u64, _i32, _u32, _dbl = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint32, ctypes.c_double
+_pd = ctypes.POINTER(ctypes.c_double)
+_pu64 = ctypes.POINTER(ctypes.c_uint64)
+SIGNATURES = {
+    "qsv_init": (_i32, [_i32]),
+    "qsv_shutdown": (_i32, []),
+    "qsv_device_query": (_i32, [ctypes.POINTER(QsvDeviceInfo)]),
+    "qsv_last_error": (ctypes.c_char_p, []),
+    "qsv_version": (ctypes.c_char_p, []),
+    "qsv_state_create": (_i32, [_i32, _i32, ctypes.POINTER(_vp)]),
+    "qsv_state_destroy": (_i32, [_vp]),
+    "qsv_state_num_qubits": (_i32, [_vp]),
+    "qsv_state_precision": (_i32, [_vp]),
+    "qsv_state_init_basis": (_i32, [_vp, _u64]),
+    "qsv_state_upload": (_i32, [_vp, _vp, _u64, _u64]),
+    "qsv_state_download": (_i32, [_vp, _vp, _u64, _u64]),
+    "qsv_state_copy": (_i32, [_vp, _vp]),
+    "qsv_state_sync": (_i32, [_vp]),
+    "qsv_apply_program": (_i32, [_vp, _vp, _u64, _u32, _u64]),
+    "qsv_norm2": (_i32, [_vp, _pd]),
+    "qsv_normalize": (_i32, [_vp]),
+    "qsv_probabilities": (_i32, [_vp, _u64, _u64, _vp]),
+    "qsv_sample": (_i32, [_vp, _u64, _u64, _vp]),
+    "qsv_measure_qubits": (_i32, [_vp, _u64, _dbl, _pu64, _vp]),
+    "qsv_expect_pauli": (_i32, [_vp, ctypes.c_char_p, _u64, _vp]),
+    "qsv_expect_1q": (_i32, [_vp, _i32, _vp, _vp]),
+    "qsv_inner": (_i32, [_vp, _vp, _vp]),
+    "qsv_uniform": (_dbl, [_u64, _u64]),
+    "qsv_batch_expect_z": (_i32, [ctypes.POINTER(QsvBatchDesc), _vp]),
+    "qsv_timing_enable": (_i32, [_vp, _i32]),
+    "qsv_timing_read": (_i32, [_vp, _vp, _i32]),
+}
+
+
+class QsvError(RuntimeError):
+    """A libqsv call returned a non-zero status."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"libqsv error {code}: {message}")
+        self.code = code
+        self.message = message
+
+
+_lib = None
+
+
+def load_library(path: str | None = None) -> ctypes.CDLL:
+    """Load libqsv.so (once) and attach the C signatures.
+
+    Raises ``ImportError`` when the library has not been built: there is no
+    fallback engine.
+    """
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise ImportError(
+            f"libqsv.so not found at {p}: build it first with "
+            "`python -c 'import __graft_entry__ as g; g.build()'` (hipcc, gfx950)")
+    lib = ctypes.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(code: int) -> None:
+    if code != QSV_OK:
+        msg = _lib.qsv_last_error().decode() if _lib is not None else ""
+        raise QsvError(code, msg)
+
+
+def ops_array(n: int) -> np.ndarray:
+    return np.zeros(n, dtype=OP_DTYPE)

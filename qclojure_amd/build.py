@@ -1,0 +1,45 @@
+"""Build libqsv.so in-tree with hipcc for gfx950.
+
+    python -m qclojure_amd.build
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "libqsv.so")
+ARCH = os.environ.get("QSV_OFFLOAD_ARCH", "gfx950")
+
+SOURCES = ["qsv_api.hip", "qsv_kernels.hip", "qsv_batch.hip"]
+
+HIPCC_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # no packed-FP32 instructions in device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-result", "-Wno-unused-value",
+]
+
+
+def build(verbose: bool = False) -> str:
+    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    tmp = OUT + ".tmp"
+    cmd = ["hipcc", *HIPCC_FLAGS, "-o", tmp, *srcs]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building libqsv.so")
+    noise = [l for l in (r.stdout + r.stderr).splitlines()
+             if l.strip() and "packed-fp32-ops" not in l and "not a recognized feature" not in l]
+    if verbose and noise:
+        print("\n".join(noise))
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,74 @@
+// qsv_internal.h — shared between the kernel file and the C-ABI file.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include "../../include/qsv.h"
+
+namespace qsv {
+
+constexpr int kBlock = 256;          // threads per workgroup (4 waves of 64)
+constexpr int kReduceBlocks = 1024;  // fixed grid for FP64 reductions
+constexpr int kMaxPauliPerPass = 8;  // Pauli terms evaluated per pass over the state
+constexpr int kSampleChunkLog = 10;  // amplitudes per chunk in the sampling prefix
+
+// Zero-bit insertion plan: enumerate `count` indices j and map each to the
+// full index by inserting zeros at the ascending bit positions ins[0..nins)
+// and OR-ing `orMask` (bits forced to one, e.g. controls).
+struct InsArgs {
+  uint64_t count;
+  uint64_t orMask;
+  int32_t nins;
+  int32_t ins[5];
+};
+
+template <typename R>
+struct Mat2 {
+  R m[8];  // row-major 2x2, (re, im) interleaved
+};
+
+template <typename R>
+struct Cplx {
+  R re, im;
+};
+
+// ---- launchers (qsv_kernels.hip) -------------------------------------------
+// All take a raw device pointer to the amplitude array (double2* for c128,
+// float2* for c64, passed as void*) and launch on `st`.
+void launch_pair(int prec, void* psi, const InsArgs& a, uint64_t tbit, const double m[8], hipStream_t st);
+void launch_diag(int prec, void* psi, const InsArgs& a, uint64_t tbit, const double d0[2], const double d1[2],
+                 hipStream_t st);
+void launch_swap(int prec, void* psi, const InsArgs& a, uint64_t bitA, uint64_t bitB, const double ph[2],
+                 hipStream_t st);
+void launch_blockade(int prec, void* psi, uint64_t S, uint64_t bmask, const double ph[2], hipStream_t st);
+void launch_fill_basis(int prec, void* psi, uint64_t S, uint64_t idx, hipStream_t st);
+void launch_scale(int prec, void* psi, uint64_t S, double f, hipStream_t st);
+void launch_f64_to_c64(const double* src, void* dst, uint64_t count, hipStream_t st);
+void launch_c64_to_f64(const void* src, double* dst, uint64_t count, hipStream_t st);
+
+// FP64 reductions. `partials` holds kReduceBlocks * nterms * 2 doubles; the
+// final values land in out[0..nterms*2) (re, im) on the device.
+void launch_norm2(int prec, const void* psi, uint64_t S, double* partials, double* out, hipStream_t st);
+void launch_inner(int prec, const void* a, const void* b, uint64_t S, double* partials, double* out,
+                  hipStream_t st);
+void launch_pauli(int prec, const void* psi, uint64_t S, int nterms, const uint64_t* xmask, const uint64_t* smask,
+                  double* partials, double* out, hipStream_t st);
+void launch_expect1q(int prec, const void* psi, uint64_t S, uint64_t tbit, const double m[8], double* partials,
+                     double* out, hipStream_t st);
+void launch_probs(int prec, const void* psi, uint64_t offset, uint64_t count, double* out, hipStream_t st);
+
+// Sampling: chunk sums -> inclusive prefix -> one wave per shot.
+void launch_sample(int prec, const void* psi, uint64_t S, double* chunk_prefix, uint64_t shots, uint64_t seed,
+                   uint64_t* out, hipStream_t st);
+
+// Measurement of a qubit subset: outcome histogram (bits gathered at positions
+// pos[0..m)), then collapse.
+void launch_outcome_hist(int prec, const void* psi, uint64_t S, int m, const int* pos, double* hist,
+                         hipStream_t st);
+void launch_collapse(int prec, void* psi, uint64_t S, int m, const int* pos, uint64_t sel, double scale,
+                     hipStream_t st);
+
+// Fused LDS-tiled pass (qsv_fused.hip).
+struct TileGate;  // defined in qsv_fused.hip
+}  // namespace qsv

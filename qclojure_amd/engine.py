@@ -1,0 +1,189 @@
+"""Device-resident state vectors over libqsv's C ABI.
+
+A :class:`DeviceState` owns one ``qsv_state`` handle: 2^n amplitudes in HBM on
+their own HIP stream. Every method is one C-ABI call; nothing here computes
+amplitudes on the host.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+
+import numpy as np
+
+from . import _abi
+from ._abi import QSV_C128, QSV_C64, check
+
+_init_lock = threading.Lock()
+_initialised = False
+
+
+def lib():
+    return _abi.load_library()
+
+
+def init(device: int = 0) -> None:
+    global _initialised
+    with _init_lock:
+        if not _initialised:
+            check(lib().qsv_init(device))
+            _initialised = True
+
+
+def device_info() -> dict:
+    init()
+    info = _abi.QsvDeviceInfo()
+    check(lib().qsv_device_query(ctypes.byref(info)))
+    return {"name": info.name.decode(), "arch": info.arch.decode(), "compute-units": info.compute_units,
+            "wavefront-size": info.wavefront_size, "hbm-bytes": info.hbm_bytes,
+            "hbm-free-bytes": info.hbm_free_bytes}
+
+
+def _ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+PRECISIONS = {"c128": QSV_C128, "complex128": QSV_C128, "c64": QSV_C64, "complex64": QSV_C64,
+              QSV_C128: QSV_C128, QSV_C64: QSV_C64}
+
+
+class DeviceState:
+    """One state vector in HBM (qsv_state_create / qsv_state_destroy)."""
+
+    def __init__(self, num_qubits: int, precision="c128"):
+        init()
+        self.num_qubits = int(num_qubits)
+        self.precision = PRECISIONS[precision]
+        h = ctypes.c_void_p()
+        check(lib().qsv_state_create(self.num_qubits, self.precision, ctypes.byref(h)))
+        self._h = h
+
+    # -- lifecycle ------------------------------------------------------------
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        if self._h is None:
+            raise RuntimeError("state already destroyed")
+        return self._h
+
+    def destroy(self) -> None:
+        if getattr(self, "_h", None) is not None:
+            lib().qsv_state_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
+
+    @property
+    def size(self) -> int:
+        return 1 << self.num_qubits
+
+    @property
+    def nbytes(self) -> int:
+        return self.size * (16 if self.precision == QSV_C128 else 8)
+
+    # -- data movement ---------------------------------------------------------
+    def init_basis(self, index: int = 0) -> "DeviceState":
+        check(lib().qsv_state_init_basis(self.handle, int(index)))
+        return self
+
+    def upload(self, amps, offset: int = 0) -> "DeviceState":
+        a = np.ascontiguousarray(np.asarray(amps, dtype=np.complex128))
+        check(lib().qsv_state_upload(self.handle, _ptr(a), offset, a.size))
+        return self
+
+    def download(self, offset: int = 0, count: int | None = None) -> np.ndarray:
+        count = self.size - offset if count is None else count
+        out = np.empty(count, dtype=np.complex128)
+        check(lib().qsv_state_download(self.handle, _ptr(out), offset, count))
+        return out
+
+    def copy_from(self, other: "DeviceState") -> "DeviceState":
+        check(lib().qsv_state_copy(self.handle, other.handle))
+        return self
+
+    def sync(self) -> None:
+        check(lib().qsv_state_sync(self.handle))
+
+    # -- gates -------------------------------------------------------------------
+    def apply(self, ops: np.ndarray, flags: int = 0, seed: int = 0) -> "DeviceState":
+        ops = np.ascontiguousarray(ops, dtype=_abi.OP_DTYPE)
+        check(lib().qsv_apply_program(self.handle, _ptr(ops), len(ops), flags, seed & (2**64 - 1)))
+        return self
+
+    # -- results -----------------------------------------------------------------
+    def norm2(self) -> float:
+        v = ctypes.c_double()
+        check(lib().qsv_norm2(self.handle, ctypes.byref(v)))
+        return v.value
+
+    def normalize(self) -> "DeviceState":
+        check(lib().qsv_normalize(self.handle))
+        return self
+
+    def probabilities(self, offset: int = 0, count: int | None = None) -> np.ndarray:
+        count = self.size - offset if count is None else count
+        out = np.empty(count, dtype=np.float64)
+        check(lib().qsv_probabilities(self.handle, offset, count, _ptr(out)))
+        return out
+
+    def sample(self, shots: int, seed: int) -> np.ndarray:
+        out = np.empty(int(shots), dtype=np.uint64)
+        check(lib().qsv_sample(self.handle, int(shots), seed & (2**64 - 1), _ptr(out)))
+        return out
+
+    def measure_qubits(self, qubits, u: float):
+        mask = 0
+        for q in qubits:
+            mask |= 1 << int(q)
+        m = bin(mask).count("1")
+        probs = np.empty(1 << m, dtype=np.float64)
+        oc = ctypes.c_uint64()
+        check(lib().qsv_measure_qubits(self.handle, mask, float(u), ctypes.byref(oc), _ptr(probs)))
+        return int(oc.value), probs
+
+    def expect_pauli(self, pauli_strings) -> np.ndarray:
+        strs = list(pauli_strings)
+        out = np.empty(len(strs), dtype=np.float64)
+        if not strs:
+            return out
+        for s in strs:
+            if len(s) != self.num_qubits:
+                raise ValueError(f"pauli string {s!r} has length {len(s)}, state has {self.num_qubits} qubits")
+        blob = "".join(strs).encode()
+        check(lib().qsv_expect_pauli(self.handle, blob, len(strs), _ptr(out)))
+        return out
+
+    def expect_1q(self, qubit: int, matrix) -> complex:
+        m = np.ascontiguousarray(np.asarray(matrix, dtype=np.complex128).reshape(4)).view(np.float64)
+        out = np.empty(2, dtype=np.float64)
+        check(lib().qsv_expect_1q(self.handle, int(qubit), _ptr(m), _ptr(out)))
+        return complex(out[0], out[1])
+
+    def inner(self, other: "DeviceState") -> complex:
+        out = np.empty(2, dtype=np.float64)
+        check(lib().qsv_inner(self.handle, other.handle, _ptr(out)))
+        return complex(out[0], out[1])
+
+    # -- timing ------------------------------------------------------------------
+    def timing(self, enable: bool) -> None:
+        check(lib().qsv_timing_enable(self.handle, 1 if enable else 0))
+
+    def timing_read(self, reset: bool = True) -> dict:
+        arr = (_abi.QsvKernelStat * 5)()
+        check(lib().qsv_timing_read(self.handle, arr, 1 if reset else 0))
+        return {name: {"launches": arr[i].launches, "total_ms": arr[i].total_ms, "bytes": arr[i].bytes}
+                for i, name in enumerate(_abi.KCLASS_NAMES)}
+
+
+def uniform(seed: int, k: int) -> float:
+    """Counter-based uniform shared with the device sampler (qsv_uniform)."""
+    return lib().qsv_uniform(seed & (2**64 - 1), k)

@@ -1,0 +1,251 @@
+"""Writes tests/golden/reference_kats.json: the known-answer vectors held by the
+reference's own test suite for the gate-application and result path.
+
+Every case records its source (file:line in lsolbach/qclojure's test tree),
+the initial state, the operations (as circuit operation maps) and the checks
+the reference test makes, with the reference's tolerance. The expected values
+are transcribed from those tests; nothing here is computed by the engine or
+the oracle.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+
+S2 = 1.0 / math.sqrt(2.0)
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def basis(n, idx):
+    v = [[0.0, 0.0] for _ in range(1 << n)]
+    v[idx] = [1.0, 0.0]
+    return v
+
+
+def op(t, **p):
+    d = {"operation-type": t}
+    if p:
+        d["operation-params"] = {k.replace("_", "-"): v for k, v in p.items()}
+    return d
+
+
+def state_eq(vec, tol=0.0):
+    return {"check": "state", "value": vec, "tol": tol}
+
+
+def amp(idx, re, im, tol=0.0):
+    return {"check": "amplitude", "index": idx, "value": [re, im], "tol": tol}
+
+
+def absval(idx, value, tol):
+    return {"check": "abs", "index": idx, "value": value, "tol": tol}
+
+
+def abs_gt(idx, value):
+    return {"check": "abs-greater", "index": idx, "value": value}
+
+
+def im_eq(idx, value, tol=0.0):
+    return {"check": "imag", "index": idx, "value": value, "tol": tol}
+
+
+def re_eq(idx, value, tol=0.0):
+    return {"check": "real", "index": idx, "value": value, "tol": tol}
+
+
+def changed():
+    return {"check": "state-changed"}
+
+
+GT = "test/org/soulspace/qclojure/domain/gate_test.clj"
+CASES = []
+
+
+def case(cid, src, n, init, ops, checks):
+    CASES.append({"id": cid, "source": src, "n": n, "init": init, "ops": ops, "checks": checks})
+
+
+# gate_test.clj:15-49 Pauli X / Z / HZH
+case("x0", GT + ":17-22", 1, basis(1, 0), [op("x", target=0)], [state_eq(basis(1, 1))])
+case("x1", GT + ":17-24", 1, basis(1, 1), [op("x", target=0)], [state_eq(basis(1, 0))])
+case("z0", GT + ":26-32", 1, basis(1, 0), [op("z", target=0)], [state_eq(basis(1, 0))])
+case("z1", GT + ":26-34", 1, basis(1, 1), [op("z", target=0)], [state_eq([[0.0, 0.0], [-1.0, 0.0]])])
+case("hzh0", GT + ":36-46", 1, basis(1, 0), [op("h", target=0), op("z", target=0), op("h", target=0)],
+     [state_eq(basis(1, 1), 1e-10)])
+case("hzh1", GT + ":36-49", 1, basis(1, 1), [op("h", target=0), op("z", target=0), op("h", target=0)],
+     [state_eq(basis(1, 0), 1e-10)])
+# :51-73 Hadamard
+case("h0", GT + ":51-61", 1, basis(1, 0), [op("h", target=0)], [state_eq([[S2, 0.0], [S2, 0.0]], 1e-10)])
+case("hh0", GT + ":63-73", 1, basis(1, 0), [op("h", target=0), op("h", target=0)], [state_eq(basis(1, 0), 1e-10)])
+case("hh1", GT + ":63-73", 1, basis(1, 1), [op("h", target=0), op("h", target=0)], [state_eq(basis(1, 1), 1e-10)])
+# :75-97 CNOT truth table (control 0, target 1)
+for src_idx, dst_idx in [(0, 0), (1, 1), (2, 3), (3, 2)]:
+    case(f"cnot{src_idx:02b}", GT + ":75-97", 2, basis(2, src_idx), [op("cnot", control=0, target=1)],
+         [state_eq(basis(2, dst_idx))])
+# :99-116 Bell
+case("bell", GT + ":99-116", 2, basis(2, 0), [op("h", target=0), op("cnot", control=0, target=1)],
+     [amp(0, S2, 0.0, 1e-10), absval(1, 0.0, 1e-10), absval(2, 0.0, 1e-10), amp(3, S2, 0.0, 1e-10)])
+# :125-145 Rydberg CZ
+for i in range(3):
+    case(f"rydcz{i:02b}", GT + ":125-143", 2, basis(2, i), [op("rydberg-cz", control=0, target=1)],
+         [state_eq(basis(2, i))])
+case("rydcz11", GT + ":125-145", 2, basis(2, 3), [op("rydberg-cz", control=0, target=1)], [amp(3, -1.0, 0.0)])
+# :147-160 Rydberg CPhase(pi/4)
+case("rydcp00", GT + ":147-158", 2, basis(2, 0), [op("rydberg-cphase", control=0, target=1, angle=math.pi / 4)],
+     [state_eq(basis(2, 0))])
+case("rydcp11", GT + ":147-160", 2, basis(2, 3), [op("rydberg-cphase", control=0, target=1, angle=math.pi / 4)],
+     [amp(3, math.cos(math.pi / 4), math.sin(math.pi / 4), 1e-10)])
+# :162-174 Rydberg blockade
+case("blk00", GT + ":162-172", 2, basis(2, 0), [op("rydberg-blockade", qubit_indices=[0, 1], angle=math.pi / 4)],
+     [state_eq(basis(2, 0))])
+case("blk01", GT + ":162-174", 2, basis(2, 1), [op("rydberg-blockade", qubit_indices=[0, 1], angle=math.pi / 4)],
+     [changed()])
+# :176-208 global gates
+case("globalx", GT + ":176-183", 3, basis(3, 0), [op("global-x")], [absval(7, 1.0, 1e-10)])
+case("globalh", GT + ":185-194", 2, basis(2, 0), [op("global-h")], [absval(i, 0.5, 1e-10) for i in range(4)])
+case("globalrx", GT + ":196-205", 2, basis(2, 0), [op("global-rx", angle=math.pi / 4)], [changed()])
+case("globalry", GT + ":196-206", 2, basis(2, 0), [op("global-ry", angle=math.pi / 4)], [changed()])
+case("globalrz", GT + ":196-208", 2, basis(2, 0), [op("global-rz", angle=math.pi / 4)], [absval(0, 1.0, 1e-10)])
+# :210-222 SWAP, :224-234 iSWAP
+case("swap01", GT + ":210-220", 2, basis(2, 1), [op("swap", qubit1=0, qubit2=1)], [state_eq(basis(2, 2))])
+case("swap10", GT + ":210-222", 2, basis(2, 2), [op("swap", qubit1=0, qubit2=1)], [state_eq(basis(2, 1))])
+case("iswap01", GT + ":224-233", 2, basis(2, 1), [op("iswap", qubit1=0, qubit2=1)], [im_eq(2, 1.0)])
+case("iswap10", GT + ":224-234", 2, basis(2, 2), [op("iswap", qubit1=0, qubit2=1)], [im_eq(1, 1.0)])
+# :236-248 Toffoli
+for s, d in [(0, 0), (6, 7), (7, 6)]:
+    case(f"toffoli{s:03b}", GT + ":236-248", 3, basis(3, s), [op("toffoli", control1=0, control2=1, target=2)],
+         [state_eq(basis(3, d))])
+# :250-280 Fredkin full table and self-inverse
+for s, d in [(0, 0), (1, 1), (2, 2), (3, 3), (4, 4), (5, 6), (6, 5), (7, 7)]:
+    case(f"fredkin{s:03b}", GT + ":250-269", 3, basis(3, s), [op("fredkin", control=0, target1=1, target2=2)],
+         [state_eq(basis(3, d))])
+    case(f"fredkin2x{s:03b}", GT + ":271-280", 3, basis(3, s),
+         [op("fredkin", control=0, target1=1, target2=2), op("fredkin", control=0, target1=1, target2=2)],
+         [state_eq(basis(3, s))])
+# :282-308 Pauli-Y
+case("y0", GT + ":284-293", 1, basis(1, 0), [op("y", target=0)], [absval(0, 0.0, 1e-10), im_eq(1, 1.0, 1e-10)])
+case("y1", GT + ":284-297", 1, basis(1, 1), [op("y", target=0)], [im_eq(0, -1.0, 1e-10), absval(1, 0.0, 1e-10)])
+case("yy0", GT + ":299-308", 1, basis(1, 0), [op("y", target=0), op("y", target=0)], [state_eq(basis(1, 0), 1e-10)])
+case("yy1", GT + ":299-308", 1, basis(1, 1), [op("y", target=0), op("y", target=0)], [state_eq(basis(1, 1), 1e-10)])
+# :310-331 controlled-Y via apply-controlled-gate (the dispatcher reaches it through :controlled :y)
+CY = op("controlled", control_qubits=[0], target_gate="y", target_qubits=[1])
+case("cy00", GT + ":312-319", 2, basis(2, 0), [CY], [state_eq(basis(2, 0))])
+case("cy01", GT + ":312-320", 2, basis(2, 1), [CY], [state_eq(basis(2, 1))])
+case("cy10", GT + ":312-326", 2, basis(2, 2), [CY], [absval(2, 0.0, 1e-10), abs_gt(3, 0.9)])
+case("cy11", GT + ":312-331", 2, basis(2, 3), [CY], [abs_gt(2, 0.9), absval(3, 0.0, 1e-10)])
+# :333-350 controlled-Z
+for i in range(3):
+    case(f"cz{i:02b}", GT + ":333-346", 2, basis(2, i), [op("cz", control=0, target=1)], [state_eq(basis(2, i))])
+case("cz11", GT + ":333-350", 2, basis(2, 3), [op("cz", control=0, target=1)], [re_eq(3, -1.0, 1e-10)])
+# :352-390 phase gates
+case("s0", GT + ":353-362", 1, basis(1, 0), [op("s", target=0)], [state_eq(basis(1, 0))])
+case("s1", GT + ":353-364", 1, basis(1, 1), [op("s", target=0)], [amp(1, 0.0, 1.0)])
+case("t0", GT + ":366-376", 1, basis(1, 0), [op("t", target=0)], [state_eq(basis(1, 0))])
+case("t1", GT + ":366-378", 1, basis(1, 1), [op("t", target=0)],
+     [amp(1, math.sqrt(2) / 2, math.sqrt(2) / 2, 1e-10)])
+case("sdgs1", GT + ":380-388", 1, basis(1, 1), [op("s", target=0), op("s-dag", target=0)], [amp(1, 1.0, 0.0, 1e-10)])
+case("tdgt1", GT + ":380-390", 1, basis(1, 1), [op("t", target=0), op("t-dag", target=0)], [amp(1, 1.0, 0.0, 1e-10)])
+# :392-431 rotations at pi/2
+case("rx0", GT + ":393-403", 1, basis(1, 0), [op("rx", target=0, angle=math.pi / 2)],
+     [amp(0, S2, 0.0, 1e-10), amp(1, 0.0, -S2, 1e-10)])
+case("ry0", GT + ":405-415", 1, basis(1, 0), [op("ry", target=0, angle=math.pi / 2)],
+     [amp(0, S2, 0.0, 1e-10), amp(1, S2, 0.0, 1e-10)])
+case("rz0", GT + ":417-428", 1, basis(1, 0), [op("rz", target=0, angle=math.pi / 2)],
+     [amp(0, math.sqrt(2) / 2, -math.sqrt(2) / 2, 1e-10)])
+case("rz1", GT + ":417-431", 1, basis(1, 1), [op("rz", target=0, angle=math.pi / 2)],
+     [amp(1, math.sqrt(2) / 2, math.sqrt(2) / 2, 1e-10)])
+# :433-452 controlled rotations
+for g in ("crx", "cry"):
+    case(f"{g}00", GT + ":433-448", 2, basis(2, 0), [op(g, control=0, target=1, angle=math.pi / 2)],
+         [state_eq(basis(2, 0))])
+    case(f"{g}10", GT + ":433-452", 2, basis(2, 2), [op(g, control=0, target=1, angle=math.pi / 2)], [changed()])
+# state_test.clj:555-568 bit order: X on qubit 0 of |00> gives index 2 (|10>)
+case("bitorder10", "test/org/soulspace/qclojure/domain/state_test.clj:555-568", 2, basis(2, 0),
+     [op("x", target=0)], [amp(2, 1.0, 0.0)])
+# circuit_test.clj:410-430 Bell via execute-circuit
+case("bell-exec", "test/org/soulspace/qclojure/domain/circuit_test.clj:410-430", 2, basis(2, 0),
+     [op("h", target=0), op("cnot", control=0, target=1)],
+     [state_eq([[S2, 0.0], [0.0, 0.0], [0.0, 0.0], [S2, 0.0]], 1e-10)])
+
+# hamiltonian_test.clj:66-115 expectation values ("exact" = the reference compares with =)
+HT = "test/org/soulspace/qclojure/domain/hamiltonian_test.clj"
+PLUS = [[S2, 0.0], [S2, 0.0]]
+MINUS = [[S2, 0.0], [-S2, 0.0]]
+EXPECT = [
+    ("Z|0>", HT + ":74", 1, basis(1, 0), [[1.0, "Z"]], 1.0, "exact"),
+    ("Z|1>", HT + ":75", 1, basis(1, 1), [[1.0, "Z"]], -1.0, "exact"),
+    ("Z|+>", HT + ":76", 1, PLUS, [[1.0, "Z"]], 0.0, 1e-10),
+    ("X|0>", HT + ":79", 1, basis(1, 0), [[1.0, "X"]], 0.0, 1e-10),
+    ("X|1>", HT + ":80", 1, basis(1, 1), [[1.0, "X"]], 0.0, 1e-10),
+    ("X|+>", HT + ":81", 1, PLUS, [[1.0, "X"]], 1.0, 1e-10),
+    ("X|->", HT + ":82", 1, MINUS, [[1.0, "X"]], -1.0, 1e-10),
+    ("I|0>", HT + ":85", 1, basis(1, 0), [[1.0, "I"]], 1.0, 1e-10),
+    ("I|1>", HT + ":86", 1, basis(1, 1), [[1.0, "I"]], 1.0, 1e-10),
+    ("I|+>", HT + ":87", 1, PLUS, [[1.0, "I"]], 1.0, 1e-10),
+    ("ZZ|00>", HT + ":93", 2, basis(2, 0), [[1.0, "ZZ"]], 1.0, "exact"),
+    ("ZZ|11>", HT + ":94", 2, basis(2, 3), [[1.0, "ZZ"]], 1.0, "exact"),
+    ("II|00>", HT + ":95", 2, basis(2, 0), [[1.0, "II"]], 1.0, "exact"),
+    ("ZI|00>", HT + ":96", 2, basis(2, 0), [[1.0, "ZI"]], 1.0, "exact"),
+    ("2I+Z|0>", HT + ":100-103", 1, basis(1, 0), [[2.0, "I"], [1.0, "Z"]], 3.0, "exact"),
+    ("2I+Z|1>", HT + ":100-104", 1, basis(1, 1), [[2.0, "I"], [1.0, "Z"]], 1.0, "exact"),
+    ("multi|00>", HT + ":106-115", 2, basis(2, 0), [[1.0, "II"], [0.5, "ZI"], [-0.3, "IZ"], [0.2, "ZZ"]],
+     ((1.0 + 0.5) + -0.3) + 0.2, "exact"),
+]
+
+# ideal_simulator_test.clj shot windows (statistical; counts of outcomes)
+IT = "test/org/soulspace/qclojure/adapter/backend/ideal_simulator_test.clj"
+SHOTS = [
+    {"id": "bell-100", "source": IT + ":58-86", "n": 2, "ops": [op("h", target=0), op("cnot", control=0, target=1)],
+     "shots": 100, "allowed": [0, 3], "distinct": 2},
+    {"id": "bell-50", "source": IT + ":88-99", "n": 2, "ops": [op("h", target=0), op("cnot", control=0, target=1)],
+     "shots": 50, "allowed": [0, 3], "must-contain": [0, 3]},
+    {"id": "h-1000", "source": IT + ":101-118", "n": 1, "ops": [op("h", target=0)], "shots": 1000,
+     "allowed": [0, 1], "ratio-of": [0, 1], "ratio-window": [0.40, 0.60]},
+    {"id": "bell-1000", "source": IT + ":120-137", "n": 2, "ops": [op("h", target=0), op("cnot", control=0, target=1)],
+     "shots": 1000, "allowed": [0, 3], "ratio-of": [0, 3], "ratio-window": [0.45, 0.55]},
+    {"id": "ghz3-1000", "source": IT + ":139-155", "n": 3,
+     "ops": [op("h", target=0), op("cnot", control=0, target=1), op("cnot", control=0, target=2)],
+     "shots": 1000, "allowed": [0, 7], "ratio-of": [0, 7], "ratio-window": [0.45, 0.55]},
+    {"id": "x-100", "source": IT + ":157-171", "n": 1, "ops": [op("x", target=0)], "shots": 100,
+     "allowed": [1], "count": [1, 100]},
+    {"id": "xcnot-100", "source": IT + ":172-185", "n": 2, "ops": [op("x", target=0), op("cnot", control=0, target=1)],
+     "shots": 100, "allowed": [3], "count": [3, 100]},
+]
+
+# qaoa_test.clj:17-33 MaxCut Hamiltonian construction
+QT = "test/org/soulspace/qclojure/application/algorithm/qaoa_test.clj"
+MAXCUT = [
+    {"id": "triangle", "source": QT + ":17-25", "graph": [[0, 1, 1.0], [1, 2, 1.0], [0, 2, 1.0]], "n": 3,
+     "num-terms": 6},
+    {"id": "single-edge-w2", "source": QT + ":27-33", "graph": [[0, 1, 2.0]], "n": 2, "num-terms": 2,
+     "coefficient-set": [1.0, -1.0]},
+]
+
+# state_test.clj:197-226 bits-to-index
+BITS = [([0], 0), ([1], 1), ([0, 0], 0), ([0, 1], 1), ([1, 0], 2), ([1, 1], 3),
+        ([0, 0, 0], 0), ([0, 0, 1], 1), ([0, 1, 0], 2), ([0, 1, 1], 3), ([1, 0, 0], 4),
+        ([1, 0, 1], 5), ([1, 1, 0], 6), ([1, 1, 1], 7), ([1, 0, 1, 1], 11), ([], 0)]
+
+
+def main():
+    data = {
+        "about": "Known-answer vectors transcribed from lsolbach/qclojure's test suite.",
+        "gate-cases": CASES,
+        "expectation-cases": [{"id": i, "source": s, "n": n, "init": st, "hamiltonian": h, "value": v, "tol": t}
+                              for (i, s, n, st, h, v, t) in EXPECT],
+        "shot-cases": SHOTS,
+        "maxcut-cases": MAXCUT,
+        "bits-to-index": [{"bits": b, "index": i, "source": "test/org/soulspace/qclojure/domain/state_test.clj:197-226"}
+                          for b, i in BITS],
+    }
+    path = os.path.join(HERE, "reference_kats.json")
+    with open(path, "w") as f:
+        json.dump(data, f, indent=1)
+    print(f"wrote {path}: {len(CASES)} gate cases, {len(EXPECT)} expectation cases, {len(SHOTS)} shot cases")
+
+
+if __name__ == "__main__":
+    main()

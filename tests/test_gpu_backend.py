@@ -1,0 +1,145 @@
+"""The QuantumBackend protocol over libqsv, following the reference's
+ideal_simulator_test.clj and result_test.clj."""
+import time
+
+import numpy as np
+import pytest
+
+from qclojure_amd import algorithms as alg
+from qclojure_amd import backend as be
+from qclojure_amd import circuit as qc
+from qclojure_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def sim():
+    be.reset_simulator_state()   # reset-simulator-fixture (ideal_simulator_test.clj:12-20)
+    s = be.create_simulator({"seed": 2024})
+    yield s
+    s.close()
+    be.reset_simulator_state()
+
+
+def test_backend_info_and_availability(sim):
+    info = sim.backend_info()
+    assert info["backend-type"] == "simulator"
+    assert "quantum-backend" in info["capabilities"]
+    assert sim.is_available() is True
+    assert sim.device()["technology"] == "simulator"
+
+
+def test_submission_and_tracking(sim):
+    job = sim.submit_circuit(qc.bell_state_circuit(), {"result-specs": {"measurements": {"shots": 100}}})
+    assert isinstance(job, str)
+    res = be.wait_for_job(sim, job, 30000)
+    assert sim.job_status(job) == "completed"
+    r = sim.job_result(job)
+    assert r["job-id"] == job and r["job-status"] == "completed"
+    m = r["results"]["measurement-results"]
+    assert set(m["frequencies"]) == {0, 3} and m["shot-count"] == 100
+    assert len(m["measurement-outcomes"]) == 100
+    assert np.allclose(np.asarray(m["measurement-probabilities"]), [0.5, 0, 0, 0.5], atol=1e-15)
+    assert res["job-status"] == "completed"
+
+
+def test_execute_circuit_result_shape(sim):
+    r = be.execute_circuit(sim, qc.bell_state_circuit(),
+                           {"result-specs": {"measurements": {"shots": 50}, "state-vector": True,
+                                             "probabilities": {}, "amplitudes": {"basis-states": [0, 3]}}})
+    assert r["job-status"] == "completed"
+    res = r["results"]
+    assert res["circuit-metadata"] == {"circuit-depth": 2, "circuit-operation-count": 2, "circuit-gate-count": 2}
+    assert res["result-types"] == {"measurements", "state-vector", "probabilities", "amplitudes"}
+    sv = np.asarray(res["final-state"]["state-vector"])
+    assert np.allclose(sv, [2 ** -0.5, 0, 0, 2 ** -0.5], atol=1e-12)
+    assert list(res["state-vector-result"]["basis-labels"]) == ["|00⟩", "|01⟩", "|10⟩", "|11⟩"]
+    assert abs(res["amplitude-results"]["amplitude-values"][3] - 2 ** -0.5) < 1e-12
+    assert abs(res["probability-results"]["probability-outcomes"][0] - 0.5) < 1e-12
+    assert "execution-time-ms" in r
+
+
+def test_hamiltonian_result(sim):
+    inst = wl.qaoa_instance(11, n_choices=(8,))
+    r = be.execute_circuit(sim, inst["circuit"], {"result-specs": {"hamiltonian": inst["hamiltonian"]}})
+    hr = r["results"]["hamiltonian-result"]
+    assert 0.0 <= hr["energy-expectation"] <= len(inst["graph"])
+    assert sum(len(g) for g in hr["measurement-groups"]) == len(inst["hamiltonian"])
+
+
+def test_expectation_variance_fidelity(sim):
+    Z = [[1, 0], [0, -1]]
+    X = [[0, 1], [1, 0]]
+    plus = {"num-qubits": 1, "state-vector": [2 ** -0.5, 2 ** -0.5]}
+    c = qc.h_gate(qc.create_circuit(1), 0)
+    r = be.execute_circuit(sim, c, {"result-specs": {
+        "expectation": {"observables": [Z, X], "targets": [0, 0]},
+        "variance": {"observables": [Z], "targets": [0]},
+        "fidelity": {"references": [plus]}}})
+    res = r["results"]
+    assert abs(res["expectation-results"][0]["expectation-value"]) < 1e-12
+    assert abs(res["expectation-results"][1]["expectation-value"] - 1.0) < 1e-12
+    assert abs(res["variance-results"][0]["variance-value"] - 1.0) < 1e-12
+    assert abs(res["fidelity-results"]["fidelities"]["reference-0"] - 1.0) < 1e-12
+
+
+def test_initial_state_option(sim):
+    init = {"num-qubits": 2, "state-vector": [0, 0, 1, 0]}   # |10>
+    r = be.execute_circuit(sim, qc.cnot_gate(qc.create_circuit(2), 0, 1),
+                           {"initial-state": init, "result-specs": {"state-vector": True}})
+    assert np.allclose(np.asarray(r["results"]["final-state"]["state-vector"]), [0, 0, 0, 1])
+
+
+def test_failed_jobs_never_hang(sim):
+    bad = {"num-qubits": 2, "operations": [{"operation-type": "cy", "operation-params": {"control": 0, "target": 1}}]}
+    r = be.execute_circuit(sim, bad, {})
+    assert r["job-status"] == "failed" and "Unknown gate type" in r["error-message"]
+    oob = {"num-qubits": 2, "operations": [{"operation-type": "h", "operation-params": {"target": 7}}]}
+    job = sim.submit_circuit(oob, {})
+    t0 = time.monotonic()
+    while sim.job_status(job) in ("queued", "running") and time.monotonic() - t0 < 30:
+        time.sleep(0.005)
+    assert sim.job_status(job) == "failed"
+    assert sim.job_result(job)["exception-type"] == "CircuitError"
+
+
+def test_not_found_and_cancel(sim):
+    r = sim.job_result("non-existent-id")
+    assert r["job-status"] == "not-found" and r["job-id"] == "non-existent-id" and "error-message" in r
+    assert sim.cancel_job("non-existent-id") == "not-found"
+    job = sim.submit_circuit(qc.ghz_state_circuit(5), {"shots": 5000})
+    assert sim.cancel_job(job) in ("cancelled", "cannot-cancel")
+
+
+def test_queue_status_and_stats(sim):
+    sim.submit_circuit(qc.bell_state_circuit(), {})
+    sim.submit_circuit(qc.ghz_state_circuit(3), {})
+    q = sim.queue_status()
+    assert q["total-jobs"] >= 2 and {"queued", "running", "completed"} <= set(q)
+    be.execute_circuit(sim, qc.bell_state_circuit(), {"shots": 100})
+    st = be.get_simulator_stats()
+    assert {"total-jobs", "completed-jobs", "average-execution-time", "job-counter"} <= set(st)
+
+
+def test_batch_job_backend(sim):
+    circs = [qc.ghz_state_circuit(n) for n in (2, 3, 4, 5)]
+    bid = sim.batch_submit(circs, {"result-specs": {"measurements": {"shots": 64}}})
+    t0 = time.monotonic()
+    while sim.batch_status(bid)["status"] != "completed" and time.monotonic() - t0 < 30:
+        time.sleep(0.005)
+    res = sim.batch_results(bid)
+    assert len(res) == 4
+    for (jid, r), n in zip(res.items(), (2, 3, 4, 5)):
+        assert set(r["results"]["measurement-results"]["frequencies"]) <= {0, (1 << n) - 1}
+
+
+def test_results_same_alone_and_concurrent(sim):
+    """Jobs running together on the device give the results they give alone."""
+    circs = [wl.random_layer_circuit(n, 6, seed=n) for n in (10, 12, 14, 16)]
+    spec = {"result-specs": {"state-vector": True}}
+    alone = [np.asarray(be.execute_circuit(sim, c, spec)["results"]["final-state"]["state-vector"]) for c in circs]
+    ids = [sim.submit_circuit(c, spec) for c in circs]
+    mixed = [np.asarray(be.wait_for_job(sim, j)["results"]["final-state"]["state-vector"]) for j in ids]
+    for a, m in zip(alone, mixed):
+        assert np.array_equal(a, m)

@@ -1,0 +1,266 @@
+"""Parity of the HIP path (libqsv through its C ABI) with the FP64 oracle and
+the reference test suite's known answers. Runs on the MI355X."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from qclojure_amd import algorithms as alg
+from qclojure_amd import circuit as qc
+from qclojure_amd import workloads as wl
+from qclojure_amd._abi import OP, QSV_FLAG_MSB_SWAP
+from kat_checks import as_vec, check_expectation, check_shots, run_checks
+
+pytestmark = pytest.mark.gpu
+
+TOL_C128 = 1e-12          # amplitude tolerance, complex128 path
+FID_C64 = 1.0 - 1e-6      # fidelity floor, complex64 path
+
+
+def note(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def eng():
+    from qclojure_amd import engine
+    engine.init(0)
+    note(f"device: {engine.device_info()}")
+    return engine
+
+
+def run_dev(eng, n, ops, init=None, prec="c128", flags=0, seed=0):
+    with eng.DeviceState(n, prec) as s:
+        if init is not None:
+            s.upload(init)
+        s.apply(ops, flags, seed)
+        return s.download()
+
+
+def test_reference_gate_kats(eng, kats):
+    for case in kats["gate-cases"]:
+        note(f"kat {case['id']}")
+        init = as_vec(case["init"])
+        ops = qc.decode_circuit({"num-qubits": case["n"], "operations": case["ops"]})
+        psi = run_dev(eng, case["n"], ops, init)
+        run_checks(case, psi, init)
+
+
+def test_reference_expectation_kats(eng, kats):
+    from qclojure_amd.hamiltonian import hamiltonian_expectation
+    for case in kats["expectation-cases"]:
+        note(f"expect {case['id']}")
+        with eng.DeviceState(case["n"]) as s:
+            s.upload(as_vec(case["init"]))
+            h = [{"coefficient": c, "pauli-string": p} for c, p in case["hamiltonian"]]
+            check_expectation(case, hamiltonian_expectation(h, s))
+
+
+def _random_program(rng, n, count, kinds1=None):
+    kinds1 = kinds1 or ["X", "Y", "Z", "H", "S", "SDG", "T", "TDG", "RX", "RY", "RZ", "PHASE"]
+    kinds2 = ["CNOT", "CZ", "CY", "CRX", "CRY", "CRZ", "RYDBERG_CZ", "RYDBERG_CPHASE", "SWAP", "ISWAP"]
+    rows = []
+    for _ in range(count):
+        r = rng.uniform()
+        ang = float(rng.uniform(0, 2 * math.pi))
+        if r < 0.5 or n < 3:
+            rows.append((OP[rng.choice(kinds1)], int(rng.integers(n)), 0, 0, 0, ang))
+        elif r < 0.85:
+            a, b = rng.choice(n, 2, replace=False)
+            rows.append((OP[rng.choice(kinds2)], int(a), int(b), 0, 0, ang))
+        elif r < 0.95:
+            a, b, c = rng.choice(n, 3, replace=False)
+            rows.append((OP[rng.choice(["TOFFOLI", "FREDKIN"])], int(a), int(b), int(c), 0, 0.0))
+        else:
+            rows.append((OP[rng.choice(["RYDBERG_BLOCKADE", "GLOBAL_RX", "GLOBAL_H", "GLOBAL_X"])],
+                         0, 0, 0, int(rng.integers(1, 1 << min(n, 62))), ang))
+    return qc.ops_from_tuples(rows)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 7, 9, 11, 14, 17])
+def test_random_programs_vs_oracle(eng, oracle, n):
+    rng = np.random.default_rng(100 + n)
+    ops = _random_program(rng, n, 120)
+    note(f"random program n={n}")
+    exp = oracle.run(n, ops)
+    got = run_dev(eng, n, ops)
+    assert np.max(np.abs(got - exp)) < TOL_C128
+    got_msb = run_dev(eng, n, ops, flags=QSV_FLAG_MSB_SWAP)
+    assert np.max(np.abs(got_msb - oracle.run(n, ops, flags=QSV_FLAG_MSB_SWAP))) < TOL_C128
+
+
+def test_literal_oracle_small(eng, oracle):
+    rng = np.random.default_rng(4)
+    for n in (2, 4, 6, 8):
+        ops = _random_program(rng, n, 40)
+        note(f"literal n={n}")
+        assert np.max(np.abs(run_dev(eng, n, ops) - oracle.run(n, ops, literal=True))) < TOL_C128
+
+
+def test_every_target_position(eng, oracle):
+    """Each 1-qubit and controlled kernel at every bit position of a 13-qubit state
+    (covers in-wave, in-block and cross-block partners)."""
+    n = 13
+    rng = np.random.default_rng(8)
+    rows = []
+    for q in range(n):
+        rows.append((OP["RX"], q, 0, 0, 0, 0.3 + q))
+        rows.append((OP["H"], q, 0, 0, 0, 0.0))
+        rows.append((OP["RZ"], q, 0, 0, 0, 1.1 * q))
+        rows.append((OP["T"], q, 0, 0, 0, 0.0))
+        c = int(rng.integers(n - 1))
+        c = c if c < q else c + 1
+        rows.append((OP["CRY"], c, q, 0, 0, 0.7 + q))
+        rows.append((OP["CZ"], c, q, 0, 0, 0.0))
+        rows.append((OP["CNOT"], q, c, 0, 0, 0.0))
+    ops = qc.ops_from_tuples(rows)
+    init = oracle.run(n, _random_program(rng, n, 30))
+    exp = oracle.apply_program(init, n, ops)
+    assert np.max(np.abs(run_dev(eng, n, ops, init) - exp)) < TOL_C128
+
+
+def test_c1_config_vs_literal_oracle(eng, oracle):
+    """12 qubits, 20 layers, complex128 from |0..0> against the dense-Kronecker
+    restatement, then 1024 shots against the exact distribution."""
+    ops = wl.random_layer_ops(12, 20, seed=1)
+    exp = oracle.run(12, ops, literal=True)
+    with eng.DeviceState(12) as s:
+        s.apply(ops)
+        got = s.download()
+        assert np.max(np.abs(got - exp)) < TOL_C128
+        shots = s.sample(1024, seed=77)
+        # device sampling follows the same inverse-CDF rule as the oracle
+        ref = oracle.sample(exp, 12, 1024, seed=77)
+        assert np.mean(shots == ref) > 0.999
+        p = np.abs(exp) ** 2
+        counts = np.bincount(shots.astype(np.int64), minlength=4096)
+        tvd = 0.5 * np.sum(np.abs(counts / 1024 - p))
+        assert tvd < 0.5  # 4096 outcomes from 1024 shots: TVD of a correct sampler ~0.35
+    # a second check with more shots and a chi-square over coarse bins
+    with eng.DeviceState(12) as s:
+        s.apply(ops)
+        shots = s.sample(200000, seed=5)
+    bins = 64
+    pb = p.reshape(bins, -1).sum(axis=1)
+    cb = np.bincount((shots.astype(np.int64) >> 6), minlength=bins)
+    chi2 = np.sum((cb - 200000 * pb) ** 2 / (200000 * pb))
+    assert chi2 < 120  # 63 dof, p ~ 1e-5
+
+
+def test_shot_windows(eng, kats):
+    for case in kats["shot-cases"]:
+        ops = qc.decode_circuit({"num-qubits": case["n"], "operations": case["ops"]})
+        with eng.DeviceState(case["n"]) as s:
+            s.apply(ops)
+            check_shots(case, s.sample(case["shots"], seed=99))
+
+
+def test_sampling_matches_oracle_rule(eng, oracle):
+    rng = np.random.default_rng(12)
+    for n in (1, 3, 10, 15, 18):
+        psi = oracle.run(n, _random_program(rng, n, 50))
+        with eng.DeviceState(n) as s:
+            s.upload(psi)
+            got = s.sample(4000, seed=31 + n)
+        exp = oracle.sample(psi, n, 4000, seed=31 + n)
+        assert np.mean(got == exp) > 0.999, n
+
+
+def test_measure_collapse_vs_oracle(eng, oracle):
+    rng = np.random.default_rng(13)
+    for n, qubits in ((3, [0]), (6, [1, 4]), (10, [0, 3, 9]), (16, list(range(0, 16, 2)))):
+        psi = oracle.run(n, _random_program(rng, n, 40))
+        mask = sum(1 << q for q in qubits)
+        for u in (0.05, 0.5, 0.93):
+            o_exp, p_exp, st_exp = oracle.measure_qubits(psi, n, mask, u)
+            with eng.DeviceState(n) as s:
+                s.upload(psi)
+                o, p = s.measure_qubits(qubits, u)
+                st = s.download()
+            assert o == o_exp
+            assert np.max(np.abs(p - p_exp)) < 1e-13
+            assert np.max(np.abs(st - st_exp)) < TOL_C128
+    # mid-circuit measurement ops inside a program use qsv_uniform(seed, k)
+    n = 5
+    rows = [(OP["H"], q, 0, 0, 0, 0.0) for q in range(n)] + [(OP["MEASURE"], 0, 0, 0, 0b10101, 0.0)] + \
+           [(OP["RY"], q, 0, 0, 0, 0.4 * q) for q in range(n)] + [(OP["MEASURE"], 0, 0, 0, 0b00010, 0.0)]
+    ops = qc.ops_from_tuples(rows)
+    assert np.max(np.abs(run_dev(eng, n, ops, seed=4242) - oracle.run(n, ops, seed=4242))) < TOL_C128
+
+
+def test_expectations_vs_oracle(eng, oracle):
+    rng = np.random.default_rng(14)
+    for n in (1, 4, 9, 16):
+        psi = oracle.run(n, _random_program(rng, n, 40))
+        strings = ["".join(rng.choice(list("IXYZ"), size=n)) for _ in range(19)] + ["I" * n, "Z" * n]
+        with eng.DeviceState(n) as s:
+            s.upload(psi)
+            got = s.expect_pauli(strings)
+            for k, st in enumerate(strings):
+                assert abs(got[k] - oracle.pauli_expectation(psi, n, st)) < 1e-12, st
+            o = rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2))
+            for q in range(n):
+                assert abs(s.expect_1q(q, o) - oracle.expect_1q(psi, n, q, o)) < 1e-12
+            assert abs(s.norm2() - oracle.norm2(psi, n)) < 1e-13
+            assert np.max(np.abs(s.probabilities() - oracle.probabilities(psi, n))) < 1e-15
+            phi = oracle.run(n, _random_program(rng, n, 10))
+            with eng.DeviceState(n) as t:
+                t.upload(phi)
+                assert abs(s.inner(t) - oracle.inner(psi, phi, n)) < 1e-12
+
+
+def test_c64_fidelity(eng, oracle):
+    """complex64 path on the 20-qubit random-layer circuit: fidelity >= 1 - 1e-6
+    against the FP64 oracle."""
+    n = 20
+    ops = wl.random_layer_ops(n, 20, seed=3)
+    exp = oracle.run(n, ops)
+    got = run_dev(eng, n, ops, prec="c64")
+    fid = abs(np.vdot(exp, got)) ** 2
+    note(f"c64 fidelity {1 - fid:.3e} max|d| {np.max(np.abs(got - exp)):.3e}")
+    assert fid > FID_C64
+    assert np.max(np.abs(got - exp)) < 1e-5
+
+
+def test_large_state_round_trip(eng):
+    """26 qubits: U then U^-1 (self-inverse gates plus negated rotations) returns
+    |0..0> to 1e-12; the norm stays 1."""
+    n = 26
+    ops = wl.random_layer_ops(n, 4, seed=17)
+    inv = ops[::-1].copy()
+    for r in inv:
+        if int(r["kind"]) in (OP["RX"], OP["RY"], OP["RZ"]):
+            r["angle"] = -r["angle"]
+    with eng.DeviceState(n) as s:
+        s.apply(ops)
+        assert abs(s.norm2() - 1.0) < 1e-12
+        s.apply(inv)
+        head = s.download(0, 1024)
+        assert abs(head[0] - 1.0) < 1e-12 and np.max(np.abs(head[1:])) < 1e-12
+        assert abs(s.norm2() - 1.0) < 1e-12
+
+
+def test_qaoa_p1_closed_form_on_device(eng):
+    """Triangle-free 3-regular graphs, p = 1: <H_C> = m (1/2 + 1/2 sin4b sin g cos^2 g)."""
+    from qclojure_amd.hamiltonian import hamiltonian_expectation
+    petersen_outer = [[i, (i + 1) % 5, 1.0] for i in range(5)]
+    petersen = petersen_outer + [[i, i + 5, 1.0] for i in range(5)] + [[5 + i, 5 + (i + 2) % 5, 1.0] for i in range(5)]
+    g, b = 0.9, 0.35
+    hp = alg.max_cut_hamiltonian(petersen, 10)
+    circ = alg.qaoa_ansatz_circuit(hp, alg.standard_mixer_hamiltonian(10), [g, b], 10)
+    with eng.DeviceState(10) as s:
+        s.apply(qc.decode_circuit(circ))
+        e = hamiltonian_expectation(hp, s)
+    assert abs(e - 15 * (0.5 + 0.5 * math.sin(4 * b) * math.sin(g) * math.cos(g) ** 2)) < 1e-12
+
+
+def test_errors_are_reported(eng):
+    from qclojure_amd._abi import QsvError
+    with eng.DeviceState(3) as s:
+        with pytest.raises(QsvError, match="out of range"):
+            s.apply(qc.ops_from_tuples([(OP["H"], 5, 0, 0, 0, 0.0)]))
+        with pytest.raises(QsvError, match="Unknown gate type"):
+            s.apply(qc.ops_from_tuples([(999, 0, 0, 0, 0, 0.0)]))
+        with pytest.raises(QsvError):
+            s.download(0, 100)
