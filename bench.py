@@ -67,6 +67,39 @@ def cpu_baseline(n: int, seed: int) -> dict:
                       f"{dt:.2f} s on {cpu_model()}"}
 
 
+def position_sweep(engine, n: int, prec: str, reps: int = 3) -> dict:
+    """GB/s of each per-gate kernel class at every target bit position
+    (bit b = n-1-q for reference qubit q), timed with HIP events."""
+    from qclojure_amd import circuit as qc
+    from qclojure_amd._abi import OP
+    dev = engine.DeviceState(n, prec)
+    table = {}
+    gates = {"H": lambda q: [(OP["H"], q, 0, 0, 0, 0.0)],
+             "RZ": lambda q: [(OP["RZ"], q, 0, 0, 0, 0.37)],
+             "T": lambda q: [(OP["T"], q, 0, 0, 0, 0.0)],
+             "CNOT(c=q-1)": lambda q: [(OP["CNOT"], (q - 1) % n, q, 0, 0, 0.0)],
+             "CZ(c=q-1)": lambda q: [(OP["CZ"], (q - 1) % n, q, 0, 0, 0.0)],
+             "SWAP(q,q-1)": lambda q: [(OP["SWAP"], q, (q - 1) % n, 0, 0, 0.0)]}
+    for name, mk in gates.items():
+        row = {}
+        for q in range(n):
+            ops = qc.ops_from_tuples(mk(q) * reps)
+            dev.apply(ops[:1])
+            dev.sync()
+            dev.timing(True)
+            dev.timing_read(reset=True)
+            dev.apply(ops)
+            st = dev.timing_read(reset=True)
+            dev.timing(False)
+            ms = sum(v["total_ms"] for v in st.values())
+            by = sum(v["bytes"] for v in st.values())
+            row[f"b{n - 1 - q}"] = round(by / (ms / 1e3) / 1e9, 1)
+        table[name] = row
+        log(f"sweep {name}: " + " ".join(f"{k}={v:.0f}" for k, v in row.items()))
+    dev.destroy()
+    return table
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +110,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-qubits", type=int, default=28)
+    ap.add_argument("--sweep", metavar="PATH", help="also write GB/s per gate type x bit position to PATH")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -159,6 +193,10 @@ def main():
                      "kernel": f"k_{dom}", "avg_launch_ms": round(d["total_ms"] / d["launches"], 4)},
         "kernels": kern,
     }
+    if args.sweep:
+        table = position_sweep(engine, n, args.prec)
+        with open(args.sweep, "w") as f:
+            json.dump({"qubits": n, "precision": args.prec, "unit": "GB/s", "table": table}, f, indent=1)
     if not args.no_cpu_baseline:
         log(f"bench: CPU baseline at {args.cpu_qubits} qubits ...")
         out["cpu_baseline"] = cpu_baseline(args.cpu_qubits, args.seed)

@@ -11,7 +11,7 @@ from .circuit import decode_circuit  # noqa: F401
 __version__ = "0.1.0"
 
 
-def engine():
+def load_engine():
     """Import the device engine (loads libqsv.so; raises if it is not built)."""
     from . import engine as _engine
     _engine.lib()

@@ -164,62 +164,80 @@ void gate_matrix(int kind, double angle, double out[8])
 inline bool is_diag(const double m[8]) { return m[2] == 0 && m[3] == 0 && m[4] == 0 && m[5] == 0; }
 inline bool is_one(double re, double im) { return re == 1.0 && im == 0.0; }
 
-// 1-qubit gate on bit b.
-int apply_1q_bit(qsv_state* s, int b, const double m[8])
-{
-  const uint64_t B = s->bytes_per_amp();
-  if (is_diag(m)) {
-    const double d0[2] = {m[0], m[1]}, d1[2] = {m[6], m[7]};
-    if (is_one(m[0], m[1])) {
-      InsArgs a = make_ins(s->S, {b}, 1ull << b);
-      return timed(s, QSV_KCLASS_DIAG, 2.0 * a.count * B,
-                   [&] { launch_diag(s->prec, s->amps, a, 1ull << b, d0, d1, s->stream); });
-    }
-    InsArgs a = make_ins(s->S, {}, 0);
-    return timed(s, QSV_KCLASS_DIAG, 2.0 * a.count * B,
-                 [&] { launch_diag(s->prec, s->amps, a, 1ull << b, d0, d1, s->stream); });
-  }
-  InsArgs a = make_ins(s->S, {b}, 0);
-  return timed(s, QSV_KCLASS_PAIR, 4.0 * a.count * B,
-               [&] { launch_pair(s->prec, s->amps, a, 1ull << b, m, s->stream); });
-}
+// Bits below this stay inside the wavefront (their 2^thr-amplitude runs are
+// shorter than a 128 B line); bits at or above it are enumerated by insertion.
+inline int low_bit_threshold(const qsv_state* s) { return s->prec == QSV_C128 ? 3 : 4; }
 
-// 2x2 `m` applied to target bit tb on the control-bits-set subspace.
+// 2x2 `m` applied to target bit tb where all control bits are set (no controls
+// for a plain 1-qubit gate). Diagonal matrices take the phase kernel, which
+// touches only the |1> half of the target when m00 == 1.
+// Algorithmic bytes: each touched amplitude is read once and written once.
 int apply_ctl_bits(qsv_state* s, const std::vector<int>& cbits, int tb, const double m[8])
 {
   const uint64_t B = s->bytes_per_amp();
-  uint64_t cmask = 0;
-  for (int c : cbits) cmask |= 1ull << c;
-  std::vector<int> ins = cbits;
+  const int thr = low_bit_threshold(s);
+  std::vector<int> ins;
+  uint64_t hor = 0, lpred = 0;
+  for (int c : cbits) {
+    if (c >= thr) { ins.push_back(c); hor |= 1ull << c; }
+    else lpred |= 1ull << c;
+  }
+  double touched = (double)(s->S >> cbits.size());
+  GateArgs g{};
   if (is_diag(m)) {
     const double d0[2] = {m[0], m[1]}, d1[2] = {m[6], m[7]};
     if (is_one(m[0], m[1])) {
-      ins.push_back(tb);
-      InsArgs a = make_ins(s->S, ins, cmask | (1ull << tb));
-      return timed(s, QSV_KCLASS_DIAG, 2.0 * a.count * B,
-                   [&] { launch_diag(s->prec, s->amps, a, 1ull << tb, d0, d1, s->stream); });
+      touched /= 2;
+      if (tb >= thr) { ins.push_back(tb); hor |= 1ull << tb; }
+      else lpred |= 1ull << tb;
     }
-    InsArgs a = make_ins(s->S, ins, cmask);
-    return timed(s, QSV_KCLASS_DIAG, 2.0 * a.count * B,
-                 [&] { launch_diag(s->prec, s->amps, a, 1ull << tb, d0, d1, s->stream); });
+    g.ins = make_ins(s->S, ins, hor);
+    g.lo_pred = lpred;
+    g.tbit = 1ull << tb;
+    return timed(s, QSV_KCLASS_DIAG, 2.0 * touched * B,
+                 [&] { launch_phase(s->prec, s->amps, g, d0, d1, s->stream); });
   }
-  ins.push_back(tb);
-  InsArgs a = make_ins(s->S, ins, cmask);
-  return timed(s, QSV_KCLASS_PAIR, 4.0 * a.count * B,
-               [&] { launch_pair(s->prec, s->amps, a, 1ull << tb, m, s->stream); });
+  if (tb >= thr) { ins.push_back(tb); g.tbit = 1ull << tb; }
+  else g.lo_t = 1ull << tb;
+  g.ins = make_ins(s->S, ins, hor);
+  g.lo_pred = lpred;
+  return timed(s, QSV_KCLASS_PAIR, 2.0 * touched * B,
+               [&] { launch_gate1(s->prec, s->amps, g, m, s->stream); });
 }
 
+int apply_1q_bit(qsv_state* s, int b, const double m[8]) { return apply_ctl_bits(s, {}, b, m); }
+
+// Exchange of index bits b1, b2 (times ph) where all control bits are set.
 int apply_swap_bits(qsv_state* s, const std::vector<int>& cbits, int b1, int b2, const double ph[2])
 {
   const uint64_t B = s->bytes_per_amp();
-  uint64_t cmask = 0;
-  for (int c : cbits) cmask |= 1ull << c;
-  std::vector<int> ins = cbits;
-  ins.push_back(b1);
-  ins.push_back(b2);
-  InsArgs a = make_ins(s->S, ins, cmask);
-  return timed(s, QSV_KCLASS_PERM, 4.0 * a.count * B,
-               [&] { launch_swap(s->prec, s->amps, a, 1ull << b1, 1ull << b2, ph, s->stream); });
+  const int thr = low_bit_threshold(s);
+  std::vector<int> ins;
+  uint64_t hor = 0, lpred = 0;
+  for (int c : cbits) {
+    if (c >= thr) { ins.push_back(c); hor |= 1ull << c; }
+    else lpred |= 1ull << c;
+  }
+  SwapArgs a{};
+  if (b1 < thr && b2 >= thr) std::swap(b1, b2);
+  if (b1 >= thr && b2 >= thr) {
+    ins.push_back(b1);
+    ins.push_back(b2);
+    a.hA = 1ull << b1;
+    a.hB = 1ull << b2;
+  } else if (b1 >= thr) {
+    ins.push_back(b1);
+    a.hA = 1ull << b1;
+    a.lB = 1ull << b2;
+  } else {
+    a.lA = 1ull << b1;
+    a.lB = 1ull << b2;
+  }
+  a.ins = make_ins(s->S, ins, hor);
+  a.lo_pred = lpred;
+  const double touched = (double)(s->S >> (cbits.size() + 1));
+  return timed(s, QSV_KCLASS_PERM, 2.0 * touched * B,
+               [&] { launch_swap2(s->prec, s->amps, a, ph, s->stream); });
 }
 
 // The reference's apply-controlled-gate applies the transpose of g

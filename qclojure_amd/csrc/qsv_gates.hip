@@ -1,0 +1,341 @@
+// qsv_gates.hip — per-gate streaming kernels (one launch per gate).
+//
+// A gate's bits are split at a threshold `thr` (3 for complex128, 4 for
+// complex64: the smallest bit whose 2^thr-amplitude runs fill a 128 B line):
+//   * high bits (>= thr) are enumerated by zero-bit insertion, so a thread
+//     owns the 2 (or 1) amplitudes the gate couples across them and every
+//     wave-wide load is a run of whole 128 B lines;
+//   * low bits (< thr) stay inside the wavefront: index bit b is lane bit b,
+//     a low target exchanges its partner with a lane shuffle
+//     (__shfl_xor -> ds_swizzle/DPP), a low control is a per-lane predicate.
+// Every amplitude a thread loads it also stores (unchanged where the gate is
+// the identity), so HBM sees full-line reads and writes. Loads and stores are
+// non-temporal: each byte of the state is touched once per gate.
+// All arithmetic is FP64 (complex64 states round once per amplitude per gate).
+#include "qsv_internal.h"
+
+namespace qsv {
+
+template <typename R> struct NV;
+template <> struct NV<double> { typedef double T __attribute__((ext_vector_type(2))); };
+template <> struct NV<float> { typedef float T __attribute__((ext_vector_type(2))); };
+
+template <typename T>
+__device__ __forceinline__ T ldnt(const T* p) { return __builtin_nontemporal_load(p); }
+template <typename T>
+__device__ __forceinline__ void stnt(T* p, T v) { __builtin_nontemporal_store(v, p); }
+
+__device__ __forceinline__ uint64_t expand_ins(uint64_t j, const InsArgs& a)
+{
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    if (k < a.nins) {
+      const uint64_t lo = j & ((1ull << a.ins[k]) - 1ull);
+      j = ((j ^ lo) << 1) | lo;
+    }
+  }
+  return j | a.orMask;
+}
+
+template <typename T>
+__device__ __forceinline__ T shfl_xor2(T v, int mask)
+{
+  T r;
+  r.x = __shfl_xor(v.x, mask, 64);
+  r.y = __shfl_xor(v.y, mask, 64);
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ T shfl2(T v, int src)
+{
+  T r;
+  r.x = __shfl(v.x, src, 64);
+  r.y = __shfl(v.y, src, 64);
+  return r;
+}
+
+// (m_r + i m_i) * (x + i y) + (n_r + i n_i) * (u + i v), FP64, rounded to R
+template <typename T, typename R>
+__device__ __forceinline__ T cmad2(double mr, double mi, T a, double nr, double ni, T b)
+{
+  const double ax = a.x, ay = a.y, bx = b.x, by = b.y;
+  T y;
+  y.x = (R)(mr * ax - mi * ay + nr * bx - ni * by);
+  y.y = (R)(mr * ay + mi * ax + nr * by + ni * bx);
+  return y;
+}
+
+template <typename T, typename R>
+__device__ __forceinline__ T cmul1(double dr, double di, T a)
+{
+  const double ax = a.x, ay = a.y;
+  T y;
+  y.x = (R)(dr * ax - di * ay);
+  y.y = (R)(dr * ay + di * ax);
+  return y;
+}
+
+// ---------------------------------------------------------------------------
+// Dense 2x2 (optionally controlled). TLOW=false: target bit is high (inserted),
+// each thread slot holds the pair. TLOW=true: target bit is a lane bit.
+// ---------------------------------------------------------------------------
+template <typename R, int PPT, bool TLOW>
+__global__ __launch_bounds__(kBlock) void k_gate1(typename NV<R>::T* __restrict__ psi, GateArgs g, Mat2<double> M)
+{
+  using T = typename NV<R>::T;
+  const uint64_t base = (uint64_t)blockIdx.x * (kBlock * PPT) + threadIdx.x;
+  const uint64_t count = g.ins.count;
+  uint64_t id[PPT];
+  if constexpr (!TLOW) {
+    T x0[PPT], x1[PPT];
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+      const uint64_t j = base + (uint64_t)k * kBlock;
+      id[k] = expand_ins(j, g.ins);
+      if (j < count) {
+        x0[k] = ldnt(psi + id[k]);
+        x1[k] = ldnt(psi + (id[k] | g.tbit));
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+      const uint64_t j = base + (uint64_t)k * kBlock;
+      if (j < count) {
+        T y0 = x0[k], y1 = x1[k];
+        if ((id[k] & g.lo_pred) == g.lo_pred) {
+          y0 = cmad2<T, R>(M.m[0], M.m[1], x0[k], M.m[2], M.m[3], x1[k]);
+          y1 = cmad2<T, R>(M.m[4], M.m[5], x0[k], M.m[6], M.m[7], x1[k]);
+        }
+        stnt(psi + id[k], y0);
+        stnt(psi + (id[k] | g.tbit), y1);
+      }
+    }
+  } else {
+    T x[PPT];
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+      const uint64_t j = base + (uint64_t)k * kBlock;
+      id[k] = expand_ins(j, g.ins);
+      x[k] = T{R(0), R(0)};
+      if (j < count) x[k] = ldnt(psi + id[k]);
+    }
+    const int lt = (int)g.lo_t;
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+      const T p = shfl_xor2(x[k], lt);  // all lanes take part in the exchange
+      const uint64_t j = base + (uint64_t)k * kBlock;
+      if (j < count) {
+        T y = x[k];
+        if ((id[k] & g.lo_pred) == g.lo_pred) {
+          if (id[k] & g.lo_t)
+            y = cmad2<T, R>(M.m[6], M.m[7], x[k], M.m[4], M.m[5], p);   // m10 p + m11 x
+          else
+            y = cmad2<T, R>(M.m[0], M.m[1], x[k], M.m[2], M.m[3], p);   // m00 x + m01 p
+        }
+        stnt(psi + id[k], y);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Diagonal: a_i *= (i & tbit) ? d1 : d0 where (i & lo_pred) == lo_pred; the
+// high fixed bits (controls, and the target for phase-only gates) are inserted.
+// ---------------------------------------------------------------------------
+template <typename R, int EPT>
+__global__ __launch_bounds__(kBlock) void k_phase(typename NV<R>::T* __restrict__ psi, GateArgs g, Cplx<double> d0,
+                                                   Cplx<double> d1)
+{
+  using T = typename NV<R>::T;
+  const uint64_t base = (uint64_t)blockIdx.x * (kBlock * EPT) + threadIdx.x;
+  const uint64_t count = g.ins.count;
+  T x[EPT];
+  uint64_t id[EPT];
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const uint64_t j = base + (uint64_t)k * kBlock;
+    id[k] = expand_ins(j, g.ins);
+    if (j < count) x[k] = ldnt(psi + id[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const uint64_t j = base + (uint64_t)k * kBlock;
+    if (j < count) {
+      T y = x[k];
+      if ((id[k] & g.lo_pred) == g.lo_pred) {
+        const Cplx<double> d = (id[k] & g.tbit) ? d1 : d0;
+        y = cmul1<T, R>(d.re, d.im, x[k]);
+      }
+      stnt(psi + id[k], y);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Swap of two bits A, B (phase ph on the exchanged amplitudes), optionally
+// controlled. MODE 0: both high; 1: hA high, lB low; 2: both low.
+// ---------------------------------------------------------------------------
+template <typename R, int PPT, int MODE>
+__global__ __launch_bounds__(kBlock) void k_swap2(typename NV<R>::T* __restrict__ psi, SwapArgs s, Cplx<double> ph)
+{
+  using T = typename NV<R>::T;
+  const uint64_t base = (uint64_t)blockIdx.x * (kBlock * PPT) + threadIdx.x;
+  const uint64_t count = s.ins.count;
+  uint64_t id[PPT];
+  T a[PPT], b[PPT];
+#pragma unroll
+  for (int k = 0; k < PPT; ++k) {
+    const uint64_t j = base + (uint64_t)k * kBlock;
+    id[k] = expand_ins(j, s.ins);
+    a[k] = T{R(0), R(0)};
+    b[k] = T{R(0), R(0)};
+    if (j < count) {
+      if constexpr (MODE == 0) {
+        a[k] = ldnt(psi + (id[k] | s.hA));
+        b[k] = ldnt(psi + (id[k] | s.hB));
+      } else if constexpr (MODE == 1) {
+        a[k] = ldnt(psi + id[k]);
+        b[k] = ldnt(psi + (id[k] | s.hA));
+      } else {
+        a[k] = ldnt(psi + id[k]);
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < PPT; ++k) {
+    const uint64_t j = base + (uint64_t)k * kBlock;
+    if constexpr (MODE == 0) {
+      if (j < count) {
+        T ya = a[k], yb = b[k];
+        if ((id[k] & s.lo_pred) == s.lo_pred) {
+          ya = cmul1<T, R>(ph.re, ph.im, b[k]);
+          yb = cmul1<T, R>(ph.re, ph.im, a[k]);
+        }
+        stnt(psi + (id[k] | s.hA), ya);
+        stnt(psi + (id[k] | s.hB), yb);
+      }
+    } else if constexpr (MODE == 1) {
+      // a = amplitude with hA = 0, b = with hA = 1; lB is a lane bit.
+      const T sa = shfl_xor2(a[k], (int)s.lB);
+      const T sb = shfl_xor2(b[k], (int)s.lB);
+      if (j < count) {
+        T ya = a[k], yb = b[k];
+        if ((id[k] & s.lo_pred) == s.lo_pred) {
+          if (id[k] & s.lB) ya = cmul1<T, R>(ph.re, ph.im, sb);   // (lB=1,hA=0) <- (lB=0,hA=1)
+          else yb = cmul1<T, R>(ph.re, ph.im, sa);                // (lB=0,hA=1) <- (lB=1,hA=0)
+        }
+        stnt(psi + id[k], ya);
+        stnt(psi + (id[k] | s.hA), yb);
+      }
+    } else {
+      const int ba = (lane & (int)s.lA) ? 1 : 0, bb = (lane & (int)s.lB) ? 1 : 0;
+      const int src = (ba != bb) ? (lane ^ (int)(s.lA | s.lB)) : lane;
+      const T p = shfl2(a[k], src);
+      if (j < count) {
+        T y = a[k];
+        if (ba != bb && (id[k] & s.lo_pred) == s.lo_pred) y = cmul1<T, R>(ph.re, ph.im, p);
+        stnt(psi + id[k], y);
+      }
+    }
+  }
+}
+
+// Rydberg blockade: phase where exactly one of the selected bits is set.
+template <typename R, int EPT>
+__global__ __launch_bounds__(kBlock) void k_blockade(typename NV<R>::T* __restrict__ psi, uint64_t S, uint64_t bmask,
+                                                      Cplx<double> ph)
+{
+  using T = typename NV<R>::T;
+  const uint64_t base = (uint64_t)blockIdx.x * (kBlock * EPT) + threadIdx.x;
+  T x[EPT];
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const uint64_t i = base + (uint64_t)k * kBlock;
+    if (i < S) x[k] = ldnt(psi + i);
+  }
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const uint64_t i = base + (uint64_t)k * kBlock;
+    if (i < S) {
+      T y = x[k];
+      if (__popcll(i & bmask) == 1) y = cmul1<T, R>(ph.re, ph.im, x[k]);
+      stnt(psi + i, y);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+static inline unsigned blocks_for(uint64_t count, int per_thread)
+{
+  const uint64_t per_block = (uint64_t)kBlock * per_thread;
+  return (unsigned)((count + per_block - 1) / per_block);
+}
+
+void launch_gate1(int prec, void* psi, const GateArgs& g, const double m[8], hipStream_t st)
+{
+  if (g.ins.count == 0) return;
+  Mat2<double> M;
+  for (int k = 0; k < 8; ++k) M.m[k] = m[k];
+  if (g.lo_t == 0) {
+    constexpr int PPT = 2;
+    const unsigned grid = blocks_for(g.ins.count, PPT);
+    if (prec == QSV_C128)
+      hipLaunchKernelGGL((k_gate1<double, PPT, false>), dim3(grid), dim3(kBlock), 0, st,
+                         (NV<double>::T*)psi, g, M);
+    else
+      hipLaunchKernelGGL((k_gate1<float, PPT, false>), dim3(grid), dim3(kBlock), 0, st, (NV<float>::T*)psi, g, M);
+  } else {
+    constexpr int PPT = 4;
+    const unsigned grid = blocks_for(g.ins.count, PPT);
+    if (prec == QSV_C128)
+      hipLaunchKernelGGL((k_gate1<double, PPT, true>), dim3(grid), dim3(kBlock), 0, st, (NV<double>::T*)psi, g, M);
+    else
+      hipLaunchKernelGGL((k_gate1<float, PPT, true>), dim3(grid), dim3(kBlock), 0, st, (NV<float>::T*)psi, g, M);
+  }
+}
+
+void launch_phase(int prec, void* psi, const GateArgs& g, const double d0[2], const double d1[2], hipStream_t st)
+{
+  if (g.ins.count == 0) return;
+  constexpr int EPT = 4;
+  const unsigned grid = blocks_for(g.ins.count, EPT);
+  const Cplx<double> c0{d0[0], d0[1]}, c1{d1[0], d1[1]};
+  if (prec == QSV_C128)
+    hipLaunchKernelGGL((k_phase<double, EPT>), dim3(grid), dim3(kBlock), 0, st, (NV<double>::T*)psi, g, c0, c1);
+  else
+    hipLaunchKernelGGL((k_phase<float, EPT>), dim3(grid), dim3(kBlock), 0, st, (NV<float>::T*)psi, g, c0, c1);
+}
+
+void launch_swap2(int prec, void* psi, const SwapArgs& s, const double ph[2], hipStream_t st)
+{
+  if (s.ins.count == 0) return;
+  const Cplx<double> p{ph[0], ph[1]};
+  const int mode = (s.hA && s.hB) ? 0 : (s.hA ? 1 : 2);
+#define QSV_SWAP_LAUNCH(MODE, PPT)                                                                              \
+  do {                                                                                                          \
+    const unsigned grid = blocks_for(s.ins.count, PPT);                                                         \
+    if (prec == QSV_C128)                                                                                       \
+      hipLaunchKernelGGL((k_swap2<double, PPT, MODE>), dim3(grid), dim3(kBlock), 0, st, (NV<double>::T*)psi, s, p); \
+    else                                                                                                        \
+      hipLaunchKernelGGL((k_swap2<float, PPT, MODE>), dim3(grid), dim3(kBlock), 0, st, (NV<float>::T*)psi, s, p);   \
+  } while (0)
+  if (mode == 0) QSV_SWAP_LAUNCH(0, 2);
+  else if (mode == 1) QSV_SWAP_LAUNCH(1, 2);
+  else QSV_SWAP_LAUNCH(2, 4);
+#undef QSV_SWAP_LAUNCH
+}
+
+void launch_blockade(int prec, void* psi, uint64_t S, uint64_t bmask, const double ph[2], hipStream_t st)
+{
+  constexpr int EPT = 4;
+  const unsigned grid = blocks_for(S, EPT);
+  const Cplx<double> p{ph[0], ph[1]};
+  if (prec == QSV_C128)
+    hipLaunchKernelGGL((k_blockade<double, EPT>), dim3(grid), dim3(kBlock), 0, st, (NV<double>::T*)psi, S, bmask, p);
+  else
+    hipLaunchKernelGGL((k_blockade<float, EPT>), dim3(grid), dim3(kBlock), 0, st, (NV<float>::T*)psi, S, bmask, p);
+}
+
+}  // namespace qsv

@@ -33,15 +33,35 @@ struct Cplx {
   R re, im;
 };
 
-// ---- launchers (qsv_kernels.hip) -------------------------------------------
-// All take a raw device pointer to the amplitude array (double2* for c128,
-// float2* for c64, passed as void*) and launch on `st`.
-void launch_pair(int prec, void* psi, const InsArgs& a, uint64_t tbit, const double m[8], hipStream_t st);
-void launch_diag(int prec, void* psi, const InsArgs& a, uint64_t tbit, const double d0[2], const double d1[2],
-                 hipStream_t st);
-void launch_swap(int prec, void* psi, const InsArgs& a, uint64_t bitA, uint64_t bitB, const double ph[2],
-                 hipStream_t st);
+// A per-gate launch: high bits are inserted (ins), low bits (< threshold) are
+// lane bits: lo_pred must all be set for the gate to act, lo_t is a low
+// target. tbit is a high target (pair kernel) or the target whose value picks
+// d0/d1 (phase kernel).
+struct GateArgs {
+  InsArgs ins;
+  uint64_t lo_pred;
+  uint64_t tbit;
+  uint64_t lo_t;
+};
+
+// Two-bit exchange: hA/hB are high bits (0 when low), lA/lB low bits.
+// Exactly one high bit is always passed as hA with the low one as lB.
+struct SwapArgs {
+  InsArgs ins;
+  uint64_t lo_pred;
+  uint64_t hA, hB;
+  uint64_t lA, lB;
+};
+
+// ---- launchers ---------------------------------------------------------------
+// All take a raw device pointer to the amplitude array (interleaved complex128
+// or complex64, passed as void*) and launch on `st`.
+// qsv_gates.hip
+void launch_gate1(int prec, void* psi, const GateArgs& g, const double m[8], hipStream_t st);
+void launch_phase(int prec, void* psi, const GateArgs& g, const double d0[2], const double d1[2], hipStream_t st);
+void launch_swap2(int prec, void* psi, const SwapArgs& s, const double ph[2], hipStream_t st);
 void launch_blockade(int prec, void* psi, uint64_t S, uint64_t bmask, const double ph[2], hipStream_t st);
+// qsv_kernels.hip
 void launch_fill_basis(int prec, void* psi, uint64_t S, uint64_t idx, hipStream_t st);
 void launch_scale(int prec, void* psi, uint64_t S, double f, hipStream_t st);
 void launch_f64_to_c64(const double* src, void* dst, uint64_t count, hipStream_t st);

@@ -1,13 +1,8 @@
-// qsv_kernels.hip — per-gate CDNA4 kernels and FP64 reductions of libqsv.
+// qsv_kernels.hip — state initialisation, conversions, FP64 reductions
+// (norm, overlaps, Pauli expectations), sampling and measurement of libqsv.
 //
 // State layout: 2^n amplitudes, interleaved (re, im); c128 = double2 (16 B),
 // c64 = float2 (8 B). Reference qubit q is index bit b = n-1-q.
-//
-// Every per-gate kernel enumerates a compressed index j and expands it by
-// zero-bit insertion (InsArgs) so that consecutive lanes touch consecutive
-// amplitudes whenever the lowest inserted bit is >= 6 (a full 64-lane wave of
-// contiguous 16 B loads); below that, each wave still consumes whole 128 B
-// lines between its paired loads, which are issued back to back.
 #include "qsv_internal.h"
 
 namespace qsv {
@@ -15,151 +10,6 @@ namespace qsv {
 template <typename R> struct VT;
 template <> struct VT<double> { using T = double2; };
 template <> struct VT<float> { using T = float2; };
-
-__device__ __forceinline__ uint64_t expand(uint64_t j, const InsArgs& a)
-{
-#pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    if (k < a.nins) {
-      const uint64_t lo = j & ((1ull << a.ins[k]) - 1ull);
-      j = ((j ^ lo) << 1) | lo;
-    }
-  }
-  return j | a.orMask;
-}
-
-static inline unsigned grid_for(uint64_t count, int per_thread)
-{
-  uint64_t per_block = (uint64_t)kBlock * per_thread;
-  return (unsigned)((count + per_block - 1) / per_block);
-}
-
-// ---------------------------------------------------------------------------
-// Dense 2x2 on index pairs (i0, i0|tbit): (a0', a1') = M (a0, a1).
-// Used for every non-diagonal 1-qubit gate and controlled 2x2 (CNOT, CY, CRX,
-// CRY, Toffoli) with controls folded into InsArgs.orMask.
-// ---------------------------------------------------------------------------
-template <typename R, int PPT>
-__global__ __launch_bounds__(kBlock) void k_pair(typename VT<R>::T* __restrict__ psi, InsArgs a, uint64_t tbit,
-                                                  Mat2<R> M)
-{
-  using T = typename VT<R>::T;
-  const uint64_t base = (uint64_t)blockIdx.x * (kBlock * PPT) + threadIdx.x;
-  T x0[PPT], x1[PPT];
-  uint64_t id[PPT];
-#pragma unroll
-  for (int k = 0; k < PPT; ++k) {
-    const uint64_t j = base + (uint64_t)k * kBlock;
-    id[k] = expand(j, a);
-    if (j < a.count) {
-      x0[k] = psi[id[k]];
-      x1[k] = psi[id[k] | tbit];
-    }
-  }
-  const R m0r = M.m[0], m0i = M.m[1], m1r = M.m[2], m1i = M.m[3];
-  const R m2r = M.m[4], m2i = M.m[5], m3r = M.m[6], m3i = M.m[7];
-#pragma unroll
-  for (int k = 0; k < PPT; ++k) {
-    const uint64_t j = base + (uint64_t)k * kBlock;
-    if (j < a.count) {
-      const T p = x0[k], q = x1[k];
-      T y0, y1;
-      y0.x = m0r * p.x - m0i * p.y + m1r * q.x - m1i * q.y;
-      y0.y = m0r * p.y + m0i * p.x + m1r * q.y + m1i * q.x;
-      y1.x = m2r * p.x - m2i * p.y + m3r * q.x - m3i * q.y;
-      y1.y = m2r * p.y + m2i * p.x + m3r * q.y + m3i * q.x;
-      psi[id[k]] = y0;
-      psi[id[k] | tbit] = y1;
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Diagonal phase: a_i *= (i & tbit) ? d1 : d0 over the enumerated subset.
-// With the target bit inserted and forced to one (orMask) only the |1>
-// subspace is touched (Z, S, T, P, CZ, CPhase).
-// ---------------------------------------------------------------------------
-template <typename R, int EPT>
-__global__ __launch_bounds__(kBlock) void k_diag(typename VT<R>::T* __restrict__ psi, InsArgs a, uint64_t tbit,
-                                                  Cplx<R> d0, Cplx<R> d1)
-{
-  using T = typename VT<R>::T;
-  const uint64_t base = (uint64_t)blockIdx.x * (kBlock * EPT) + threadIdx.x;
-  T x[EPT];
-  uint64_t id[EPT];
-#pragma unroll
-  for (int k = 0; k < EPT; ++k) {
-    const uint64_t j = base + (uint64_t)k * kBlock;
-    id[k] = expand(j, a);
-    if (j < a.count) x[k] = psi[id[k]];
-  }
-#pragma unroll
-  for (int k = 0; k < EPT; ++k) {
-    const uint64_t j = base + (uint64_t)k * kBlock;
-    if (j < a.count) {
-      const Cplx<R> d = (id[k] & tbit) ? d1 : d0;
-      T y;
-      y.x = d.re * x[k].x - d.im * x[k].y;
-      y.y = d.re * x[k].y + d.im * x[k].x;
-      psi[id[k]] = y;
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Swap-type permutation: for each enumerated base (both bits zero, controls
-// set) exchange a[base|bitA] and a[base|bitB], multiplying both by `ph`
-// (1 for SWAP/Fredkin, i for iSWAP).
-// ---------------------------------------------------------------------------
-template <typename R, int PPT>
-__global__ __launch_bounds__(kBlock) void k_swap(typename VT<R>::T* __restrict__ psi, InsArgs a, uint64_t bitA,
-                                                  uint64_t bitB, Cplx<R> ph)
-{
-  using T = typename VT<R>::T;
-  const uint64_t base = (uint64_t)blockIdx.x * (kBlock * PPT) + threadIdx.x;
-  T xa[PPT], xb[PPT];
-  uint64_t id[PPT];
-#pragma unroll
-  for (int k = 0; k < PPT; ++k) {
-    const uint64_t j = base + (uint64_t)k * kBlock;
-    id[k] = expand(j, a);
-    if (j < a.count) {
-      xa[k] = psi[id[k] | bitA];
-      xb[k] = psi[id[k] | bitB];
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < PPT; ++k) {
-    const uint64_t j = base + (uint64_t)k * kBlock;
-    if (j < a.count) {
-      T ya, yb;
-      ya.x = ph.re * xb[k].x - ph.im * xb[k].y;
-      ya.y = ph.re * xb[k].y + ph.im * xb[k].x;
-      yb.x = ph.re * xa[k].x - ph.im * xa[k].y;
-      yb.y = ph.re * xa[k].y + ph.im * xa[k].x;
-      psi[id[k] | bitA] = ya;
-      psi[id[k] | bitB] = yb;
-    }
-  }
-}
-
-// Rydberg blockade: phase where exactly one of the selected bits is set.
-template <typename R>
-__global__ __launch_bounds__(kBlock) void k_blockade(typename VT<R>::T* __restrict__ psi, uint64_t S, uint64_t bmask,
-                                                      Cplx<R> ph)
-{
-  using T = typename VT<R>::T;
-  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < S; i += stride) {
-    if (__popcll(i & bmask) == 1) {
-      const T x = psi[i];
-      T y;
-      y.x = ph.re * x.x - ph.im * x.y;
-      y.y = ph.re * x.y + ph.im * x.x;
-      psi[i] = y;
-    }
-  }
-}
 
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_fill_basis(typename VT<R>::T* __restrict__ psi, uint64_t S, uint64_t idx)
@@ -549,66 +399,6 @@ static inline unsigned stream_grid(uint64_t S)
   uint64_t g = (S + kBlock - 1) / kBlock;
   const uint64_t cap = 256ull * 64;  // 64 workgroups per CU on 256 CUs
   return (unsigned)(g < 1 ? 1 : (g > cap ? cap : g));
-}
-
-template <typename R>
-static Mat2<R> to_mat(const double m[8])
-{
-  Mat2<R> M;
-  for (int k = 0; k < 8; ++k) M.m[k] = (R)m[k];
-  return M;
-}
-
-void launch_pair(int prec, void* psi, const InsArgs& a, uint64_t tbit, const double m[8], hipStream_t st)
-{
-  if (a.count == 0) return;
-  constexpr int PPT = 4;
-  const unsigned g = grid_for(a.count, PPT);
-  if (prec == QSV_C128)
-    hipLaunchKernelGGL((k_pair<double, PPT>), dim3(g), dim3(kBlock), 0, st, (double2*)psi, a, tbit, to_mat<double>(m));
-  else
-    hipLaunchKernelGGL((k_pair<float, PPT>), dim3(g), dim3(kBlock), 0, st, (float2*)psi, a, tbit, to_mat<float>(m));
-}
-
-void launch_diag(int prec, void* psi, const InsArgs& a, uint64_t tbit, const double d0[2], const double d1[2],
-                 hipStream_t st)
-{
-  if (a.count == 0) return;
-  constexpr int EPT = 4;
-  const unsigned g = grid_for(a.count, EPT);
-  if (prec == QSV_C128) {
-    Cplx<double> c0{d0[0], d0[1]}, c1{d1[0], d1[1]};
-    hipLaunchKernelGGL((k_diag<double, EPT>), dim3(g), dim3(kBlock), 0, st, (double2*)psi, a, tbit, c0, c1);
-  } else {
-    Cplx<float> c0{(float)d0[0], (float)d0[1]}, c1{(float)d1[0], (float)d1[1]};
-    hipLaunchKernelGGL((k_diag<float, EPT>), dim3(g), dim3(kBlock), 0, st, (float2*)psi, a, tbit, c0, c1);
-  }
-}
-
-void launch_swap(int prec, void* psi, const InsArgs& a, uint64_t bitA, uint64_t bitB, const double ph[2],
-                 hipStream_t st)
-{
-  if (a.count == 0) return;
-  constexpr int PPT = 4;
-  const unsigned g = grid_for(a.count, PPT);
-  if (prec == QSV_C128) {
-    Cplx<double> p{ph[0], ph[1]};
-    hipLaunchKernelGGL((k_swap<double, PPT>), dim3(g), dim3(kBlock), 0, st, (double2*)psi, a, bitA, bitB, p);
-  } else {
-    Cplx<float> p{(float)ph[0], (float)ph[1]};
-    hipLaunchKernelGGL((k_swap<float, PPT>), dim3(g), dim3(kBlock), 0, st, (float2*)psi, a, bitA, bitB, p);
-  }
-}
-
-void launch_blockade(int prec, void* psi, uint64_t S, uint64_t bmask, const double ph[2], hipStream_t st)
-{
-  const unsigned g = stream_grid(S);
-  if (prec == QSV_C128)
-    hipLaunchKernelGGL((k_blockade<double>), dim3(g), dim3(kBlock), 0, st, (double2*)psi, S, bmask,
-                       Cplx<double>{ph[0], ph[1]});
-  else
-    hipLaunchKernelGGL((k_blockade<float>), dim3(g), dim3(kBlock), 0, st, (float2*)psi, S, bmask,
-                       Cplx<float>{(float)ph[0], (float)ph[1]});
 }
 
 void launch_fill_basis(int prec, void* psi, uint64_t S, uint64_t idx, hipStream_t st)

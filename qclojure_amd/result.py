@@ -212,33 +212,38 @@ def extract_fidelity_result(dev: DeviceState, reference_states) -> dict:
     return {"fidelities": fids, "reference-states": reference_states}
 
 
+def _truthy(v) -> bool:
+    """Clojure truthiness: only nil and false are false (an empty map is true)."""
+    return v is not None and v is not False
+
+
 def extract_results(base_result: dict, result_specs: dict, dev: DeviceState, seed: int = 0) -> dict:
     """result.clj:535-639."""
     out = dict(base_result)
     rs = result_specs or {}
-    if rs.get("measurements"):
+    if _truthy(rs.get("measurements")):
         spec = rs["measurements"]
         out["measurement-results"] = extract_measurement_results(
             dev, spec.get("qubits"), int(spec.get("shots") or 1), seed)
-    if rs.get("expectation"):
+    if _truthy(rs.get("expectation")):
         spec = rs["expectation"]
         out["expectation-results"] = extract_expectation_results(dev, spec["observables"], spec.get("targets"))
-    if rs.get("variance"):
+    if _truthy(rs.get("variance")):
         spec = rs["variance"]
         out["variance-results"] = extract_variance_results(dev, spec["observables"], spec.get("targets"))
-    if rs.get("hamiltonian"):
+    if _truthy(rs.get("hamiltonian")):
         out["hamiltonian-result"] = extract_hamiltonian_expectation(dev, rs["hamiltonian"])
-    if rs.get("probabilities"):
+    if _truthy(rs.get("probabilities")):
         spec = rs["probabilities"]
         out["probability-results"] = extract_probability_results(dev, spec.get("qubits"), spec.get("targets"))
-    if rs.get("amplitudes"):
+    if _truthy(rs.get("amplitudes")):
         out["amplitude-results"] = extract_amplitude_results(dev, rs["amplitudes"]["basis-states"])
-    if rs.get("state-vector"):
+    if _truthy(rs.get("state-vector")):
         out["state-vector-result"] = extract_state_vector_result(dev)
-    if rs.get("density-matrix"):
+    if _truthy(rs.get("density-matrix")):
         out["density-matrix-result"] = extract_density_matrix_result(dev)
-    if rs.get("fidelity"):
+    if _truthy(rs.get("fidelity")):
         out["fidelity-results"] = extract_fidelity_result(dev, rs["fidelity"]["references"])
-    if rs.get("sample"):
+    if _truthy(rs.get("sample")):
         raise NotImplementedError("the :sample result type (observable eigen-sampling) is not provided")
     return out

@@ -136,7 +136,11 @@ def test_c1_config_vs_literal_oracle(eng, oracle):
         p = np.abs(exp) ** 2
         counts = np.bincount(shots.astype(np.int64), minlength=4096)
         tvd = 0.5 * np.sum(np.abs(counts / 1024 - p))
-        assert tvd < 0.5  # 4096 outcomes from 1024 shots: TVD of a correct sampler ~0.35
+        # TVD of 1024 shots over 4096 outcomes is large even for an exact sampler:
+        # compare with numpy's multinomial sampler on the same distribution.
+        draws = np.random.default_rng(0).multinomial(1024, p / p.sum(), size=400)
+        ref_tvd = 0.5 * np.abs(draws / 1024 - p).sum(axis=1)
+        assert tvd < ref_tvd.mean() + 6 * ref_tvd.std(), (tvd, ref_tvd.mean(), ref_tvd.std())
     # a second check with more shots and a chi-square over coarse bins
     with eng.DeviceState(12) as s:
         s.apply(ops)
