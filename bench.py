@@ -109,6 +109,7 @@ def main():
     ap.add_argument("--prec", default="c128", choices=["c128", "c64"])
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fused", action="store_true", help="skip the fused tiled-pass measurement")
     ap.add_argument("--cpu-qubits", type=int, default=28)
     ap.add_argument("--sweep", metavar="PATH", help="also write GB/s per gate type x bit position to PATH")
     args = ap.parse_args()
@@ -158,6 +159,30 @@ def main():
     dom = max(stats, key=lambda k: stats[k]["total_ms"])
     d = stats[dom]
     achieved = d["bytes"] / (d["total_ms"] / 1e3) / 1e9
+    fused = None
+    if not args.no_fused:
+        from qclojure_amd._abi import QSV_FLAG_FUSE
+        for w in range(args.warmup):
+            dev.apply(layer(w), QSV_FLAG_FUSE)
+        dev.sync()
+        dev.timing(True)
+        dev.timing_read(reset=True)
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            dev.apply(layer(args.warmup + k), QSV_FLAG_FUSE)
+        dev.sync()
+        f_elapsed = time.perf_counter() - t0
+        fst = dev.timing_read(reset=True)
+        dev.timing(False)
+        blk = fst["block"]
+        fused = {"ms_per_layer": round(f_elapsed / args.steps * 1e3, 3),
+                 "speedup_vs_per_gate": round(elapsed / f_elapsed, 2),
+                 "passes_per_layer": round(blk["launches"] / args.steps, 2),
+                 "tile_kernel_avg_ms": round(blk["total_ms"] / max(1, blk["launches"]), 4),
+                 "tile_kernel_GB_s": round(blk["bytes"] / (blk["total_ms"] / 1e3) / 1e9, 1) if blk["launches"] else None,
+                 "effective_GB_s": round(total_bytes / args.steps / (f_elapsed / args.steps) / 1e9, 1),
+                 "note": "effective_GB_s = per-gate algorithmic bytes of the layer / fused layer time"}
+        log(f"bench: fused {fused}")
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
@@ -192,6 +217,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": f"k_{dom}", "avg_launch_ms": round(d["total_ms"] / d["launches"], 4)},
         "kernels": kern,
+        "fused": fused,
     }
     if args.sweep:
         table = position_sweep(engine, n, args.prec)

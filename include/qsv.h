@@ -168,6 +168,61 @@ typedef struct qsv_batch_desc {
 
 int qsv_batch_expect_z(const qsv_batch_desc* desc, double* energies);
 
+/* ---- serving: continuous batching of circuit requests ---------------------
+ * The asynchronous job model of QuantumBackend (submit-circuit / job-status /
+ * job-result, application/backend.clj:72-112; the reference runs each job in
+ * its own future, ideal_simulator.clj:120-136) as a native request loop.
+ * A scheduler thread admits queued requests in arrival order while the sum of
+ * their state bytes (2^n * 16 or 8) fits the HBM budget, keeps admitted states
+ * on the device, gathers every admitted request small enough for one
+ * workgroup's LDS into a single batched launch (gates + shots), runs larger
+ * ones concurrently on their own streams through fused passes, and records
+ * submit / admit / done timestamps inside the library. */
+typedef struct qsv_server qsv_server;
+
+typedef struct qsv_server_config {
+  uint64_t hbm_budget_bytes;  /* 0: 80% of the free device memory                 */
+  int32_t  precision;         /* QSV_C64 or QSV_C128                               */
+  int32_t  max_inflight;      /* admitted requests on the device at once (0: 32)   */
+  uint32_t flags;             /* QSV_FLAG_* applied to every request               */
+  int32_t  batch_small;       /* 1: batch LDS-sized requests into one launch       */
+} qsv_server_config;
+
+#define QSV_REQ_QUEUED    0
+#define QSV_REQ_RUNNING   1
+#define QSV_REQ_COMPLETED 2
+#define QSV_REQ_FAILED    3
+
+typedef struct qsv_request_info {
+  int32_t  status;            /* QSV_REQ_*                                          */
+  int32_t  num_qubits;
+  uint64_t shots;
+  double   queue_ms;          /* submit -> admitted                                 */
+  double   latency_ms;        /* submit -> results on the host                      */
+  char     error[128];
+} qsv_request_info;
+
+typedef struct qsv_server_stats {
+  uint64_t submitted, completed, failed;
+  uint64_t queued_now, inflight_now, max_queue_depth;
+  uint64_t small_batches, small_batched_requests;
+  uint64_t hbm_budget_bytes, hbm_in_use_bytes;
+} qsv_server_stats;
+
+int qsv_server_create(const qsv_server_config* cfg, qsv_server** out);
+/* Validates and decodes the ops (errors return immediately) and queues the request. */
+int qsv_server_submit(qsv_server* srv, int num_qubits, const qsv_op* ops, uint64_t n_ops,
+                      uint64_t shots, uint64_t seed, uint64_t* request_id);
+/* Non-blocking status (QSV_E_NOTFOUND for unknown or already collected ids). */
+int qsv_server_status(qsv_server* srv, uint64_t request_id, qsv_request_info* info);
+/* Waits up to timeout_s for completion; on completion copies `shots` outcomes
+ * (if outcomes != NULL) and forgets the request. Returns QSV_E_STATE on timeout. */
+int qsv_server_wait(qsv_server* srv, uint64_t request_id, double timeout_s,
+                    qsv_request_info* info, uint64_t* outcomes);
+int qsv_server_stats_read(qsv_server* srv, qsv_server_stats* out);
+/* Drains in-flight work, then frees every state and the scheduler thread. */
+int qsv_server_destroy(qsv_server* srv);
+
 /* ---- per-kernel timing (measurement support) ----------------------------
  * When enabled, qsv_apply_program brackets every kernel it launches with HIP
  * events on the state's own stream and accumulates (count, total ms,

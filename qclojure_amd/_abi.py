@@ -79,6 +79,24 @@ class QsvBatchDesc(ctypes.Structure):
                 ("precision", ctypes.c_int32)]
 
 
+class QsvServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("precision", ctypes.c_int32),
+                ("max_inflight", ctypes.c_int32), ("flags", ctypes.c_uint32), ("batch_small", ctypes.c_int32)]
+
+
+class QsvRequestInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("num_qubits", ctypes.c_int32), ("shots", ctypes.c_uint64),
+                ("queue_ms", ctypes.c_double), ("latency_ms", ctypes.c_double), ("error", ctypes.c_char * 128)]
+
+
+class QsvServerStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in (
+        "submitted", "completed", "failed", "queued_now", "inflight_now", "max_queue_depth",
+        "small_batches", "small_batched_requests", "hbm_budget_bytes", "hbm_in_use_bytes")]
+
+
+REQ_STATUS = {0: "queued", 1: "running", 2: "completed", 3: "failed"}
+
 # Every symbol include/qsv.h declares, with (restype, argtypes).
 _vp, _u64, _i32, _u32, _dbl = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint32, ctypes.c_double
 _pd = ctypes.POINTER(ctypes.c_double)
@@ -109,6 +127,12 @@ SIGNATURES = {
     "qsv_inner": (_i32, [_vp, _vp, _vp]),
     "qsv_uniform": (_dbl, [_u64, _u64]),
     "qsv_batch_expect_z": (_i32, [ctypes.POINTER(QsvBatchDesc), _vp]),
+    "qsv_server_create": (_i32, [ctypes.POINTER(QsvServerConfig), ctypes.POINTER(_vp)]),
+    "qsv_server_submit": (_i32, [_vp, _i32, _vp, _u64, _u64, _u64, _pu64]),
+    "qsv_server_status": (_i32, [_vp, _u64, ctypes.POINTER(QsvRequestInfo)]),
+    "qsv_server_wait": (_i32, [_vp, _u64, _dbl, ctypes.POINTER(QsvRequestInfo), _vp]),
+    "qsv_server_stats_read": (_i32, [_vp, ctypes.POINTER(QsvServerStats)]),
+    "qsv_server_destroy": (_i32, [_vp]),
     "qsv_timing_enable": (_i32, [_vp, _i32]),
     "qsv_timing_read": (_i32, [_vp, _vp, _i32]),
 }

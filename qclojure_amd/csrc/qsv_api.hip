@@ -4,11 +4,16 @@
 // apply-gate-to-state does (src/org/soulspace/qclojure/domain/circuit.clj:933-1072)
 // and turns each into one streaming kernel on the state's own HIP stream.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <mutex>
 #include <string>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "qsv_internal.h"
@@ -38,6 +43,8 @@ struct Pending {
   hipEvent_t a, b;
   double bytes;
 };
+
+void pool_release();
 }  // namespace
 
 struct qsv_state {
@@ -56,6 +63,18 @@ struct qsv_state {
   std::vector<Pending> pending;
   std::vector<hipEvent_t> evpool;
   qsv_kernel_stat stats[QSV_KCLASS_COUNT] = {};
+  // fused-pass descriptor ring (pinned host staging + device copy per slot)
+  struct PassSlot {
+    qsv::FPass* host = nullptr;
+    qsv::FPass* dev = nullptr;
+    size_t cap = 0;
+    hipEvent_t event = nullptr;
+  };
+  static constexpr int kRing = 4;
+  PassSlot ring[kRing];
+  int ring_next = 0;
+  PassSlot* ring_pending = nullptr;
+  uint64_t passes_launched = 0;
   uint64_t bytes_per_amp() const { return prec == QSV_C128 ? 16 : 8; }
 };
 
@@ -205,8 +224,6 @@ int apply_ctl_bits(qsv_state* s, const std::vector<int>& cbits, int tb, const do
                [&] { launch_gate1(s->prec, s->amps, g, m, s->stream); });
 }
 
-int apply_1q_bit(qsv_state* s, int b, const double m[8]) { return apply_ctl_bits(s, {}, b, m); }
-
 // Exchange of index bits b1, b2 (times ph) where all control bits are set.
 int apply_swap_bits(qsv_state* s, const std::vector<int>& cbits, int b1, int b2, const double ph[2])
 {
@@ -295,9 +312,59 @@ int measure_impl(qsv_state* s, uint64_t mask, double u, uint64_t* outcome, doubl
 
 bool qok(int q, int n) { return q >= 0 && q < n; }
 
-int apply_op(qsv_state* s, const qsv_op& op, uint32_t flags, uint64_t seed, uint64_t* measure_k)
+// A decoded gate in global index-bit coordinates (bit b = n-1-q).
+enum { BG_MEASURE = 10 };
+
+struct BGate {
+  int kind = FG_DENSE;     // FG_DENSE, FG_PHASE, FG_SWAP, FG_BLOCKADE or BG_MEASURE
+  std::vector<int> cbits;  // control bits (all must be 1)
+  int a = -1, b = -1;      // target bit / swap bits
+  uint64_t bmask = 0;      // blockade index bits; measured qubit mask for BG_MEASURE
+  bool phase_only = false;
+  double m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double u = 0.0;          // BG_MEASURE uniform
+};
+
+uint64_t gate_bits(const BGate& g)
 {
-  const int n = s->n;
+  uint64_t v = 0;
+  for (int c : g.cbits) v |= 1ull << c;
+  if (g.a >= 0) v |= 1ull << g.a;
+  if (g.b >= 0) v |= 1ull << g.b;
+  if (g.kind == FG_BLOCKADE) v |= g.bmask;
+  return v;
+}
+
+void emit_ctl(std::vector<BGate>& out, std::vector<int> cbits, int tb, const double m[8])
+{
+  BGate g;
+  g.cbits = std::move(cbits);
+  g.a = tb;
+  std::memcpy(g.m, m, sizeof(g.m));
+  if (is_diag(m)) {
+    g.kind = FG_PHASE;
+    g.phase_only = is_one(m[0], m[1]);
+  }
+  out.push_back(std::move(g));
+}
+
+void emit_swap(std::vector<BGate>& out, std::vector<int> cbits, int b1, int b2, double pr, double pi)
+{
+  BGate g;
+  g.kind = FG_SWAP;
+  g.cbits = std::move(cbits);
+  g.a = b1;
+  g.b = b2;
+  g.m[0] = pr;
+  g.m[1] = pi;
+  out.push_back(std::move(g));
+}
+
+// apply-gate-to-state (circuit.clj:933-1072) for one op: validate and emit
+// its gates. Controlled gates carry the transpose of the target matrix, as the
+// reference's apply-controlled-gate applies it (gate.clj:461-486).
+int decode_op(int n, const qsv_op& op, uint32_t flags, uint64_t seed, uint64_t* measure_k, std::vector<BGate>& out)
+{
   const int k = op.kind;
   double m[8], t[8];
   auto bit = [n](int q) { return n - 1 - q; };
@@ -306,10 +373,12 @@ int apply_op(qsv_state* s, const qsv_op& op, uint32_t flags, uint64_t seed, uint
   case QSV_OP_T: case QSV_OP_TDG: case QSV_OP_RX: case QSV_OP_RY: case QSV_OP_RZ: case QSV_OP_PHASE:
     if (!qok(op.q[0], n)) return fail(QSV_E_INVALID, "target qubit out of range");
     gate_matrix(k, op.angle, m);
-    return apply_1q_bit(s, bit(op.q[0]), m);
+    emit_ctl(out, {}, bit(op.q[0]), m);
+    return QSV_OK;
   case QSV_OP_U:
     if (!qok(op.q[0], n)) return fail(QSV_E_INVALID, "target qubit out of range");
-    return apply_1q_bit(s, bit(op.q[0]), op.m);
+    emit_ctl(out, {}, bit(op.q[0]), op.m);
+    return QSV_OK;
   case QSV_OP_CNOT: case QSV_OP_CZ: case QSV_OP_CY: case QSV_OP_RYDBERG_CZ:
   case QSV_OP_CRX: case QSV_OP_CRY: case QSV_OP_CRZ: {
     const int c = op.q[0], tq = op.q[1];
@@ -321,52 +390,61 @@ int apply_op(qsv_state* s, const qsv_op& op, uint32_t flags, uint64_t seed, uint
                  : (k == QSV_OP_CRY) ? QSV_OP_RY : QSV_OP_RZ;
     gate_matrix(gk, op.angle, m);
     transpose2(m, t);
-    return apply_ctl_bits(s, {bit(c)}, bit(tq), t);
+    emit_ctl(out, {bit(c)}, bit(tq), t);
+    return QSV_OK;
   }
   case QSV_OP_CU: {
     const int c = op.q[0], tq = op.q[1];
     if (!qok(c, n) || !qok(tq, n) || c == tq) return fail(QSV_E_INVALID, "control/target invalid");
-    return apply_ctl_bits(s, {bit(c)}, bit(tq), op.m);
+    emit_ctl(out, {bit(c)}, bit(tq), op.m);
+    return QSV_OK;
   }
   case QSV_OP_SWAP: case QSV_OP_ISWAP: {
     const int q1 = op.q[0], q2 = op.q[1];
     if (!qok(q1, n) || !qok(q2, n) || q1 == q2) return fail(QSV_E_INVALID, "swap qubits invalid");
+    // the reference shifts by the qubit numbers themselves (gate.clj:768-778, 823-833)
     const bool msb = (flags & QSV_FLAG_MSB_SWAP) != 0;
     const int b1 = msb ? bit(q1) : q1, b2 = msb ? bit(q2) : q2;
-    const double ph[2] = {k == QSV_OP_ISWAP ? 0.0 : 1.0, k == QSV_OP_ISWAP ? 1.0 : 0.0};
-    return apply_swap_bits(s, {}, b1, b2, ph);
+    if (k == QSV_OP_ISWAP) emit_swap(out, {}, b1, b2, 0.0, 1.0);
+    else emit_swap(out, {}, b1, b2, 1.0, 0.0);
+    return QSV_OK;
   }
   case QSV_OP_TOFFOLI: {
     const int a = op.q[0], b = op.q[1], c = op.q[2];
     if (!qok(a, n) || !qok(b, n) || !qok(c, n) || a == b || a == c || b == c)
       return fail(QSV_E_INVALID, "toffoli qubits invalid");
     gate_matrix(QSV_OP_X, 0, m);
-    return apply_ctl_bits(s, {bit(a), bit(b)}, bit(c), m);
+    emit_ctl(out, {bit(a), bit(b)}, bit(c), m);
+    return QSV_OK;
   }
   case QSV_OP_FREDKIN: {
     const int a = op.q[0], b = op.q[1], c = op.q[2];
     if (!qok(a, n) || !qok(b, n) || !qok(c, n) || a == b || a == c || b == c)
       return fail(QSV_E_INVALID, "fredkin qubits invalid");
-    const double ph[2] = {1.0, 0.0};
-    return apply_swap_bits(s, {bit(a)}, bit(b), bit(c), ph);
+    emit_swap(out, {bit(a)}, bit(b), bit(c), 1.0, 0.0);
+    return QSV_OK;
   }
   case QSV_OP_RYDBERG_CPHASE: {
     const int c = op.q[0], tq = op.q[1];
     if (!qok(c, n) || !qok(tq, n) || c == tq) return fail(QSV_E_INVALID, "control/target invalid");
     const double dm[8] = {1, 0, 0, 0, 0, 0, std::cos(op.angle), std::sin(op.angle)};
-    return apply_ctl_bits(s, {bit(c)}, bit(tq), dm);
+    emit_ctl(out, {bit(c)}, bit(tq), dm);
+    return QSV_OK;
   }
   case QSV_OP_RYDBERG_BLOCKADE: {
     if (op.mask == 0 || (n < 64 && (op.mask >> n) != 0)) return fail(QSV_E_INVALID, "blockade qubit set invalid");
-    uint64_t bm = 0;
+    BGate g;
+    g.kind = FG_BLOCKADE;
     for (int q = 0; q < n; ++q)
-      if ((op.mask >> q) & 1ull) bm |= 1ull << bit(q);
-    const double ph[2] = {std::cos(op.angle), std::sin(op.angle)};
-    return timed(s, QSV_KCLASS_DIAG, 2.0 * s->S * s->bytes_per_amp(),
-                 [&] { launch_blockade(s->prec, s->amps, s->S, bm, ph, s->stream); });
+      if ((op.mask >> q) & 1ull) g.bmask |= 1ull << bit(q);
+    g.m[0] = std::cos(op.angle);
+    g.m[1] = std::sin(op.angle);
+    out.push_back(std::move(g));
+    return QSV_OK;
   }
   case QSV_OP_GLOBAL_RX: case QSV_OP_GLOBAL_RY: case QSV_OP_GLOBAL_RZ: case QSV_OP_GLOBAL_H:
   case QSV_OP_GLOBAL_X: case QSV_OP_GLOBAL_Y: case QSV_OP_GLOBAL_Z: {
+    // global-*-gate (gate.clj:1098-1253): the gate on qubits 0..n-1 in turn
     int gk;
     double ang = op.angle;
     switch (k) {
@@ -379,19 +457,211 @@ int apply_op(qsv_state* s, const qsv_op& op, uint32_t flags, uint64_t seed, uint
     default: gk = QSV_OP_RZ; ang = M_PI; break;
     }
     gate_matrix(gk, ang, m);
-    for (int q = 0; q < n; ++q) {
-      int rc = apply_1q_bit(s, bit(q), m);
-      if (rc) return rc;
-    }
+    for (int q = 0; q < n; ++q) emit_ctl(out, {}, bit(q), m);
     return QSV_OK;
   }
   case QSV_OP_MEASURE: {
-    const double u = qsv_uniform(seed, (*measure_k)++);
-    return measure_impl(s, op.mask, u, nullptr, nullptr);
+    if (op.mask == 0 || (n < 64 && (op.mask >> n) != 0)) return fail(QSV_E_INVALID, "measure: qubit set empty or out of range");
+    BGate g;
+    g.kind = BG_MEASURE;
+    g.bmask = op.mask;
+    g.u = qsv_uniform(seed, (*measure_k)++);
+    out.push_back(std::move(g));
+    return QSV_OK;
   }
   default:
     return fail(QSV_E_UNSUPPORTED, "Unknown gate type (kind " + std::to_string(k) + ")");
   }
+}
+
+int run_direct(qsv_state* s, const BGate& g)
+{
+  switch (g.kind) {
+  case FG_DENSE:
+  case FG_PHASE:
+    return apply_ctl_bits(s, g.cbits, g.a, g.m);
+  case FG_SWAP:
+    return apply_swap_bits(s, g.cbits, g.a, g.b, g.m);
+  case FG_BLOCKADE:
+    return timed(s, QSV_KCLASS_DIAG, 2.0 * s->S * s->bytes_per_amp(),
+                 [&] { launch_blockade(s->prec, s->amps, s->S, g.bmask, g.m, s->stream); });
+  default:
+    return measure_impl(s, g.bmask, g.u, nullptr, nullptr);
+  }
+}
+
+// ---- fused passes --------------------------------------------------------
+// Pass descriptors travel through a small ring of pinned host + device
+// buffers; a slot is reused only after the event recorded behind its last
+// pass has completed.
+int upload_passes(qsv_state* s, const std::vector<FPass>& passes, FPass** dev)
+{
+  auto& slot = s->ring[s->ring_next];
+  s->ring_next = (s->ring_next + 1) % qsv_state::kRing;
+  if (slot.event) HIPCHK(hipEventSynchronize(slot.event));
+  const size_t need = passes.size();
+  if (slot.cap < need) {
+    if (slot.host) HIPCHK(hipHostFree(slot.host));
+    if (slot.dev) HIPCHK(hipFree(slot.dev));
+    slot.host = nullptr;
+    slot.dev = nullptr;
+    const size_t cap = std::max<size_t>(need, 64);
+    HIPCHK(hipHostMalloc((void**)&slot.host, cap * sizeof(FPass), hipHostMallocDefault));
+    HIPCHK(hipMalloc((void**)&slot.dev, cap * sizeof(FPass)));
+    slot.cap = cap;
+  }
+  if (!slot.event) HIPCHK(hipEventCreateWithFlags(&slot.event, hipEventDisableTiming));
+  std::memcpy(slot.host, passes.data(), need * sizeof(FPass));
+  HIPCHK(hipMemcpyAsync(slot.dev, slot.host, need * sizeof(FPass), hipMemcpyHostToDevice, s->stream));
+  *dev = slot.dev;
+  s->ring_pending = &slot;
+  return QSV_OK;
+}
+
+int launch_passes(qsv_state* s, std::vector<FPass>& passes)
+{
+  if (passes.empty()) return QSV_OK;
+  FPass* dev = nullptr;
+  int rc = upload_passes(s, passes, &dev);
+  if (rc) return rc;
+  const int TB = s->prec == QSV_C128 ? kTileBitsC128 : kTileBitsC64;
+  const uint64_t ntiles = s->S >> TB;
+  for (size_t i = 0; i < passes.size(); ++i) {
+    const FPass* p = dev + i;
+    rc = timed(s, QSV_KCLASS_BLOCK, 2.0 * s->S * s->bytes_per_amp(),
+               [&] { launch_tile(s->prec, s->amps, p, ntiles, s->stream); });
+    if (rc) return rc;
+  }
+  HIPCHK(hipEventRecord(s->ring_pending->event, s->stream));
+  s->passes_launched += passes.size();
+  passes.clear();
+  return QSV_OK;
+}
+
+// Build one pass from the chosen gates (program order) and the tile bit set.
+// Returns the number of chosen gates the pass holds (a suffix may be cut to
+// respect the stage limit).
+size_t build_pass(const qsv_state* s, const std::vector<BGate>& gs, const std::vector<size_t>& chosen,
+                  uint64_t set, FPass& p)
+{
+  const int TB = s->prec == QSV_C128 ? kTileBitsC128 : kTileBitsC64;
+  const int RB = TB - 8;
+  const int L = low_bit_threshold(s);
+  std::memset(&p, 0, sizeof(p));
+  uint64_t tile = set;
+  for (int b = 0; __builtin_popcountll(tile) < TB; ++b) tile |= 1ull << b;
+  int loc[64];
+  for (int b = 0; b < 64; ++b) loc[b] = -1;
+  int nl = 0;
+  for (int b = 0; b < L; ++b) { p.pos[nl] = b; loc[b] = nl++; }
+  for (int b = L; b < s->n; ++b)
+    if ((tile >> b) & 1ull) { p.pos[nl] = b; loc[b] = nl++; }
+  int ns = 0;
+  for (int b = 0; b < s->n; ++b)
+    if ((tile >> b) & 1ull) p.sorted_pos[ns++] = b;
+  const uint32_t lowmask = (1u << L) - 1u;
+  uint32_t cur = 0;
+  int g0 = 0, nst = 0;
+  size_t taken = 0;
+  for (size_t i = 0; i < chosen.size(); ++i) {
+    const BGate& g = gs[chosen[i]];
+    FGate fg{};
+    fg.kind = g.kind;
+    fg.a = g.a >= 0 ? loc[g.a] : -1;
+    fg.b = g.b >= 0 ? loc[g.b] : -1;
+    for (int c : g.cbits) fg.cmask |= 1u << loc[c];
+    if (g.kind == FG_BLOCKADE)
+      for (int b = 0; b < s->n; ++b)
+        if ((g.bmask >> b) & 1ull) fg.bmask |= 1u << loc[b];
+    fg.phase_only = g.phase_only ? 1 : 0;
+    std::memcpy(fg.m, g.m, sizeof(fg.m));
+    uint32_t need = 0;
+    if (g.kind == FG_DENSE) need = 1u << fg.a;
+    if (g.kind == FG_SWAP) need = (1u << fg.a) | (1u << fg.b);
+    if (__builtin_popcount(cur | need) > RB) {
+      if (nst == kMaxPassStages - 1) break;   // stage limit: cut the pass here
+      p.stages[nst++] = FStage{cur, g0, (int32_t)i};
+      cur = need;
+      g0 = (int)i;
+    } else {
+      cur |= need;
+    }
+    p.gates[i] = fg;
+    taken = i + 1;
+  }
+  p.stages[nst++] = FStage{cur, g0, (int32_t)taken};
+  p.nstages = nst;
+  p.ngates = (int)taken;
+  // fill every register mask to RB bits, high non-low tile bits first
+  for (int k = 0; k < nst; ++k) {
+    uint32_t r = p.stages[k].regmask;
+    for (int b = TB - 1; b >= 0 && __builtin_popcount(r) < RB; --b)
+      if (b >= L && !((r >> b) & 1u)) r |= 1u << b;
+    for (int b = 0; b < TB && __builtin_popcount(r) < RB; ++b) r |= 1u << b;
+    p.stages[k].regmask = r;
+  }
+  const uint32_t natural = ((1u << TB) - 1u) & ~((1u << (TB - RB)) - 1u);
+  p.load_mask = (p.stages[0].regmask & lowmask) ? natural : p.stages[0].regmask;
+  p.store_mask = (p.stages[nst - 1].regmask & lowmask) ? natural : p.stages[nst - 1].regmask;
+  return taken;
+}
+
+int run_fused(qsv_state* s, const std::vector<BGate>& gs)
+{
+  const int TB = s->prec == QSV_C128 ? kTileBitsC128 : kTileBitsC64;
+  if (s->n < TB) {
+    for (const auto& g : gs) {
+      int rc = run_direct(s, g);
+      if (rc) return rc;
+    }
+    return QSV_OK;
+  }
+  const int L = low_bit_threshold(s);
+  const uint64_t lowmask = (1ull << L) - 1ull;
+  const uint64_t all = (s->n >= 64) ? ~0ull : ((1ull << s->n) - 1ull);
+  const size_t N = gs.size();
+  std::vector<char> done(N, 0);
+  std::vector<FPass> passes;
+  std::vector<size_t> chosen;
+  size_t first = 0;
+  int rc;
+  while (true) {
+    while (first < N && done[first]) ++first;
+    if (first >= N) break;
+    const BGate& g0 = gs[first];
+    if (g0.kind == BG_MEASURE || __builtin_popcountll(gate_bits(g0) | lowmask) > TB) {
+      if ((rc = launch_passes(s, passes))) return rc;
+      if ((rc = run_direct(s, g0))) return rc;
+      done[first] = 1;
+      continue;
+    }
+    uint64_t set = lowmask, blocked = 0;
+    chosen.clear();
+    size_t scanned = 0;
+    for (size_t i = first; i < N && scanned < 4096; ++i) {
+      if (done[i]) continue;
+      ++scanned;
+      const BGate& g = gs[i];
+      if (g.kind == BG_MEASURE) break;
+      const uint64_t gb = gate_bits(g);
+      if (!(gb & blocked)) {
+        const uint64_t nu = set | gb;
+        if (__builtin_popcountll(nu) <= TB && chosen.size() < (size_t)kMaxPassGates) {
+          set = nu;
+          chosen.push_back(i);
+          continue;
+        }
+      }
+      blocked |= gb;
+      if ((blocked & all) == all) break;
+    }
+    FPass p;
+    const size_t taken = build_pass(s, gs, chosen, set, p);
+    for (size_t i = 0; i < taken; ++i) done[chosen[i]] = 1;
+    passes.push_back(p);
+    if (passes.size() >= 512 && (rc = launch_passes(s, passes))) return rc;
+  }
+  return launch_passes(s, passes);
 }
 
 int check_state(const qsv_state* s)
@@ -421,6 +691,7 @@ int qsv_init(int device)
 
 int qsv_shutdown(void)
 {
+  pool_release();
   std::lock_guard<std::mutex> lk(g_mu);
   if (g_init) hipDeviceSynchronize();
   g_init = false;
@@ -505,6 +776,11 @@ int qsv_state_destroy(qsv_state* s)
     hipEventDestroy(p.b);
   }
   for (auto e : s->evpool) hipEventDestroy(e);
+  for (auto& slot : s->ring) {
+    if (slot.host) hipHostFree(slot.host);
+    if (slot.dev) hipFree(slot.dev);
+    if (slot.event) hipEventDestroy(slot.event);
+  }
   if (s->amps) hipFree(s->amps);
   if (s->partials) hipFree(s->partials);
   if (s->dout) hipFree(s->dout);
@@ -608,14 +884,25 @@ int qsv_apply_program(qsv_state* s, const qsv_op* ops, uint64_t n_ops, uint32_t 
   int rc = check_state(s);
   if (rc) return rc;
   if (n_ops && !ops) return fail(QSV_E_INVALID, "null ops");
+  // decode and validate the whole program before launching anything
+  std::vector<BGate> gates;
+  gates.reserve(n_ops);
   uint64_t mk = 0;
   for (uint64_t i = 0; i < n_ops; ++i) {
-    rc = apply_op(s, ops[i], flags, seed, &mk);
+    rc = decode_op(s->n, ops[i], flags, seed, &mk, gates);
     if (rc) {
       g_err = "op " + std::to_string(i) + ": " + g_err;
       return rc;
     }
   }
+  if (flags & QSV_FLAG_FUSE) {
+    rc = run_fused(s, gates);
+  } else {
+    rc = QSV_OK;
+    for (const auto& g : gates)
+      if ((rc = run_direct(s, g))) break;
+  }
+  if (rc) return rc;
   HIPCHK(hipGetLastError());
   return QSV_OK;
 }
@@ -769,6 +1056,711 @@ int qsv_inner(const qsv_state* a, const qsv_state* b, double out[2])
   launch_inner(a->prec, a->amps, b->amps, a->S, a->partials, a->dout, a->stream);
   HIPCHK(hipMemcpyAsync(out, a->dout, sizeof(double) * 2, hipMemcpyDeviceToHost, a->stream));
   HIPCHK(hipStreamSynchronize(a->stream));
+  return QSV_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Pooled device states for batched circuits too large for one workgroup's
+// LDS, keyed by (num_qubits, precision); reused across batches.
+std::mutex g_pool_mu;
+std::vector<qsv_state*> g_pool[2][40];
+constexpr int kPoolPerSize = 8;
+
+int pool_get(int n, int prec, int slot, qsv_state** out)
+{
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  auto& v = g_pool[prec][n];
+  while ((int)v.size() <= slot) {
+    qsv_state* s = nullptr;
+    int rc = qsv_state_create(n, prec, &s);
+    if (rc) return rc;
+    v.push_back(s);
+  }
+  *out = v[slot];
+  return QSV_OK;
+}
+
+void pool_release()
+{
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  for (auto& per_prec : g_pool)
+    for (auto& v : per_prec) {
+      for (qsv_state* s : v) qsv_state_destroy(s);
+      v.clear();
+    }
+}
+
+template <class T>
+int dev_upload(const std::vector<T>& v, T** d, hipStream_t st)
+{
+  *d = nullptr;
+  if (v.empty()) return QSV_OK;
+  HIPCHK(hipMalloc((void**)d, v.size() * sizeof(T)));
+  HIPCHK(hipMemcpyAsync(*d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+  return QSV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
+{
+  if (!d || !energies) return fail(QSV_E_INVALID, "null argument");
+  if (d->precision != QSV_C128 && d->precision != QSV_C64) return fail(QSV_E_INVALID, "unknown precision");
+  const uint64_t N = d->n_instances;
+  if (N == 0) return QSV_OK;
+  // the pooled states are shared: batches on one process run one at a time
+  static std::mutex batch_mu;
+  std::lock_guard<std::mutex> batch_lock(batch_mu);
+  if (!d->num_qubits || !d->op_offsets || !d->term_offsets || (d->op_offsets[N] && !d->ops) ||
+      (d->term_offsets[N] && (!d->term_zmask || !d->term_coeff)))
+    return fail(QSV_E_INVALID, "batch descriptor has null arrays");
+  int rc = ensure_init();
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(g_device));
+  const int prec = d->precision;
+  const size_t B = prec == QSV_C128 ? 16 : 8;
+  const int lds_n = prec == QSV_C128 ? 13 : 14;   // 2^n * B <= 128 KiB of LDS
+
+  std::vector<BRec> recs;
+  std::vector<BInst> insts(N);
+  std::vector<uint64_t> tz;
+  std::vector<double> tc;
+  std::vector<std::vector<BGate>> big_gates;
+  std::vector<uint64_t> big_idx;
+  std::vector<int32_t> small_idx;
+  int max_small_n = 0;
+  for (uint64_t i = 0; i < N; ++i) {
+    const int n = d->num_qubits[i];
+    if (n < 1 || n > 34) return fail(QSV_E_INVALID, "instance " + std::to_string(i) + ": num_qubits out of range");
+    std::vector<BGate> gs;
+    uint64_t mk = 0;
+    for (uint64_t k = d->op_offsets[i]; k < d->op_offsets[i + 1]; ++k) {
+      if (d->ops[k].kind == QSV_OP_MEASURE)
+        return fail(QSV_E_UNSUPPORTED, "instance " + std::to_string(i) + ": measurements are not batched");
+      rc = decode_op(n, d->ops[k], 0, 0, &mk, gs);
+      if (rc) {
+        g_err = "instance " + std::to_string(i) + ", op " + std::to_string(k - d->op_offsets[i]) + ": " + g_err;
+        return rc;
+      }
+    }
+    BInst& in = insts[i];
+    in.n = n;
+    in.t0 = tz.size();
+    for (uint64_t k = d->term_offsets[i]; k < d->term_offsets[i + 1]; ++k) {
+      const uint64_t qm = d->term_zmask[k];
+      if (n < 64 && (qm >> n)) return fail(QSV_E_INVALID, "instance " + std::to_string(i) + ": Z term on a missing qubit");
+      uint64_t bm = 0;
+      for (int q = 0; q < n; ++q)
+        if ((qm >> q) & 1ull) bm |= 1ull << (n - 1 - q);
+      tz.push_back(bm);
+      tc.push_back(d->term_coeff[k]);
+    }
+    in.t1 = tz.size();
+    in.out_offset = ~0ull;
+    if (n <= lds_n) {
+      in.g0 = recs.size();
+      for (const BGate& g : gs) {
+        BRec r{};
+        r.kind = g.kind;
+        r.a = g.a;
+        r.b = g.b;
+        r.phase_only = g.phase_only ? 1 : 0;
+        for (int c : g.cbits) r.cmask |= 1ull << c;
+        r.bmask = g.bmask;
+        std::memcpy(r.m, g.m, sizeof(r.m));
+        recs.push_back(r);
+      }
+      in.g1 = recs.size();
+      small_idx.push_back((int32_t)i);
+      max_small_n = std::max(max_small_n, n);
+    } else {
+      in.g0 = in.g1 = 0;
+      big_idx.push_back(i);
+      big_gates.push_back(std::move(gs));
+    }
+  }
+  // largest circuits first, so the long workgroups start early
+  std::stable_sort(small_idx.begin(), small_idx.end(),
+                   [&](int32_t a, int32_t b) { return insts[a].n > insts[b].n; });
+
+  hipStream_t st = nullptr;
+  HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  BRec* d_recs = nullptr;
+  BInst* d_insts = nullptr;
+  uint64_t* d_tz = nullptr;
+  double* d_tc = nullptr;
+  int32_t* d_order = nullptr;
+  double* d_energy = nullptr;
+  auto cleanup = [&]() {
+    hipStreamSynchronize(st);
+    hipFree(d_recs);
+    hipFree(d_insts);
+    hipFree(d_tz);
+    hipFree(d_tc);
+    hipFree(d_order);
+    hipFree(d_energy);
+    hipStreamDestroy(st);
+  };
+  if ((rc = dev_upload(recs, &d_recs, st)) || (rc = dev_upload(insts, &d_insts, st)) ||
+      (rc = dev_upload(tz, &d_tz, st)) || (rc = dev_upload(tc, &d_tc, st)) ||
+      (rc = dev_upload(small_idx, &d_order, st))) {
+    cleanup();
+    return rc;
+  }
+  if (hipMalloc(&d_energy, N * sizeof(double)) != hipSuccess) {
+    cleanup();
+    return fail(QSV_E_NOMEM, "batch: energies allocation failed");
+  }
+  // the pooled states' streams read the uploaded terms: order them after the copies
+  hipEvent_t uploaded = nullptr;
+  if (hipEventCreateWithFlags(&uploaded, hipEventDisableTiming) != hipSuccess ||
+      hipEventRecord(uploaded, st) != hipSuccess) {
+    cleanup();
+    return fail(QSV_E_DEVICE, "batch: event creation failed");
+  }
+  if (!small_idx.empty()) {
+    const size_t lds = ((size_t)1 << max_small_n) * B;
+    if (setup_lds_batch(prec, lds) != 0) {
+      cleanup();
+      return fail(QSV_E_DEVICE, "batch: cannot reserve " + std::to_string(lds) + " B of LDS per workgroup");
+    }
+    launch_lds_batch(prec, (unsigned)small_idx.size(), lds, d_insts, d_recs, d_tz, d_tc, d_order, d_energy, nullptr,
+                     st);
+  }
+  // circuits too large for LDS: fused passes on pooled states, spread over
+  // the pool's streams so that several run at once
+  std::vector<size_t> big_order(big_idx.size());
+  for (size_t k = 0; k < big_order.size(); ++k) big_order[k] = k;
+  std::stable_sort(big_order.begin(), big_order.end(),
+                   [&](size_t a, size_t b) { return insts[big_idx[a]].n > insts[big_idx[b]].n; });
+  int slot_rr[2][40] = {};
+  std::vector<qsv_state*> used;
+  for (size_t k : big_order) {
+    const uint64_t i = big_idx[k];
+    const int n = insts[i].n;
+    qsv_state* s = nullptr;
+    const int slot = slot_rr[prec][n]++ % kPoolPerSize;
+    if ((rc = pool_get(n, prec, slot, &s))) {
+      cleanup();
+      return rc;
+    }
+    if (hipStreamWaitEvent(s->stream, uploaded, 0) != hipSuccess) {
+      cleanup();
+      return fail(QSV_E_DEVICE, "batch: stream wait failed");
+    }
+    launch_fill_basis(s->prec, s->amps, s->S, 0, s->stream);
+    if ((rc = run_fused(s, big_gates[k]))) {
+      cleanup();
+      return rc;
+    }
+    launch_zexpect(prec, s->amps, s->S, d_tz + insts[i].t0, d_tc + insts[i].t0, insts[i].t1 - insts[i].t0,
+                   s->partials, d_energy + i, s->stream);
+    used.push_back(s);
+  }
+  hipError_t e = hipGetLastError();
+  for (qsv_state* s : used)
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(energies, d_energy, N * sizeof(double), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  cleanup();
+  hipEventDestroy(uploaded);
+  if (e != hipSuccess) return fail(QSV_E_DEVICE, std::string("batch: ") + hipGetErrorString(e));
+  return QSV_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Serving: a native request loop with admission by state bytes and
+// continuous batching (see include/qsv.h).
+// ---------------------------------------------------------------------------
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct SrvReq {
+  uint64_t id = 0;
+  int n = 0;
+  std::vector<BGate> gates;
+  uint64_t shots = 0, seed = 0;
+  Clock::time_point t_submit, t_admit, t_done;
+  int status = QSV_REQ_QUEUED;
+  std::string err;
+  std::vector<uint64_t> outcomes;
+  uint64_t bytes = 0;
+  bool small = false;
+  // in flight (large requests)
+  qsv_state* st = nullptr;
+  uint64_t* d_out = nullptr;
+  uint64_t* h_out = nullptr;
+  hipEvent_t ev = nullptr;
+};
+
+// Reusable device/host buffers for one batched launch of small requests.
+struct SmallSlot {
+  void* arena = nullptr;       size_t arena_cap = 0;     // final states, back to back
+  BRec* recs = nullptr;        size_t recs_cap = 0;
+  BInst* insts = nullptr;      size_t insts_cap = 0;
+  int32_t* order = nullptr;    size_t order_cap = 0;
+  SampleJob* jobs = nullptr;   size_t jobs_cap = 0;
+  uint64_t* d_out = nullptr;   size_t out_cap = 0;
+  uint64_t* h_out = nullptr;   size_t hout_cap = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev = nullptr;
+  bool busy = false;
+  std::vector<SrvReq*> reqs;
+  std::vector<uint64_t> out_offsets;
+};
+
+template <class T>
+int grow_dev(T** p, size_t* cap, size_t need)
+{
+  if (*cap >= need) return QSV_OK;
+  if (*p) HIPCHK(hipFree(*p));
+  *p = nullptr;
+  const size_t c = std::max(need, (*cap) * 2);
+  HIPCHK(hipMalloc((void**)p, c * sizeof(T)));
+  *cap = c;
+  return QSV_OK;
+}
+
+int grow_arena(void** p, size_t* cap, size_t need)
+{
+  if (*cap >= need) return QSV_OK;
+  if (*p) HIPCHK(hipFree(*p));
+  *p = nullptr;
+  const size_t c = std::max(need, (*cap) * 2);
+  HIPCHK(hipMalloc(p, c));
+  *cap = c;
+  return QSV_OK;
+}
+
+}  // namespace
+
+struct qsv_server {
+  qsv_server_config cfg{};
+  uint64_t budget = 0, in_use = 0;
+  int lds_n = 14;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<SrvReq*> queue;
+  std::vector<SrvReq*> pending_small, pending_large, running_large;
+  std::unordered_map<uint64_t, SrvReq*> reqs;
+  uint64_t next_id = 1;
+  bool stop = false;
+  std::thread th;
+  std::vector<qsv_state*> free_states[40];
+  static constexpr int kSlots = 4;
+  SmallSlot slots[kSlots];
+  qsv_server_stats stats{};
+  std::string fatal;
+
+  int inflight() const
+  {
+    int k = (int)running_large.size() + (int)pending_large.size() + (int)pending_small.size();
+    for (const auto& s : slots)
+      if (s.busy) k += (int)s.reqs.size();
+    return k;
+  }
+};
+
+namespace {
+
+int take_state(qsv_server* sv, int n, qsv_state** out)
+{
+  auto& fl = sv->free_states[n];
+  if (!fl.empty()) {
+    *out = fl.back();
+    fl.pop_back();
+    return QSV_OK;
+  }
+  return qsv_state_create(n, sv->cfg.precision, out);
+}
+
+void give_state(qsv_server* sv, qsv_state* s)
+{
+  auto& fl = sv->free_states[s->n];
+  if (fl.size() < 2) fl.push_back(s);
+  else qsv_state_destroy(s);
+}
+
+void finish(qsv_server* sv, SrvReq* r, int status, const std::string& err)
+{
+  // called with sv->mu held
+  r->status = status;
+  r->err = err;
+  r->t_done = Clock::now();
+  sv->in_use -= r->bytes;
+  if (status == QSV_REQ_COMPLETED) sv->stats.completed++;
+  else sv->stats.failed++;
+}
+
+// Launch every pending small request as one batched launch on a free slot.
+int launch_small(qsv_server* sv, std::vector<SrvReq*>& batch)
+{
+  SmallSlot* slot = nullptr;
+  for (auto& s : sv->slots)
+    if (!s.busy) { slot = &s; break; }
+  if (!slot) return 1;  // all slots in flight; retry later
+  const int prec = sv->cfg.precision;
+  const size_t B = prec == QSV_C128 ? 16 : 8;
+  std::vector<BRec> recs;
+  std::vector<BInst> insts(batch.size());
+  std::vector<int32_t> order(batch.size());
+  std::vector<SampleJob> jobs(batch.size());
+  slot->out_offsets.assign(batch.size(), 0);
+  uint64_t arena_amps = 0, nout = 0;
+  int maxn = 0;
+  for (size_t k = 0; k < batch.size(); ++k) {
+    SrvReq* r = batch[k];
+    BInst& in = insts[k];
+    in.n = r->n;
+    in.g0 = recs.size();
+    for (const BGate& g : r->gates) {
+      BRec rec{};
+      rec.kind = g.kind;
+      rec.a = g.a;
+      rec.b = g.b;
+      rec.phase_only = g.phase_only ? 1 : 0;
+      for (int c : g.cbits) rec.cmask |= 1ull << c;
+      rec.bmask = g.bmask;
+      std::memcpy(rec.m, g.m, sizeof(rec.m));
+      recs.push_back(rec);
+    }
+    in.g1 = recs.size();
+    in.t0 = in.t1 = 0;
+    in.out_offset = arena_amps;
+    jobs[k] = SampleJob{arena_amps, nout, r->shots, r->seed, r->n, 0};
+    slot->out_offsets[k] = nout;
+    arena_amps += 1ull << r->n;
+    nout += r->shots;
+    maxn = std::max(maxn, r->n);
+    order[k] = (int32_t)k;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return insts[a].n > insts[b].n; });
+  int rc;
+  if ((rc = grow_arena(&slot->arena, &slot->arena_cap, std::max<uint64_t>(arena_amps, 1) * B))) return rc;
+  if ((rc = grow_dev(&slot->recs, &slot->recs_cap, std::max<size_t>(recs.size(), 1)))) return rc;
+  if ((rc = grow_dev(&slot->insts, &slot->insts_cap, insts.size()))) return rc;
+  if ((rc = grow_dev(&slot->order, &slot->order_cap, order.size()))) return rc;
+  if ((rc = grow_dev(&slot->jobs, &slot->jobs_cap, jobs.size()))) return rc;
+  if ((rc = grow_dev(&slot->d_out, &slot->out_cap, std::max<uint64_t>(nout, 1)))) return rc;
+  if (slot->hout_cap < nout) {
+    if (slot->h_out) HIPCHK(hipHostFree(slot->h_out));
+    slot->h_out = nullptr;
+    const size_t c = std::max<size_t>(nout, slot->hout_cap * 2);
+    HIPCHK(hipHostMalloc((void**)&slot->h_out, c * sizeof(uint64_t), hipHostMallocDefault));
+    slot->hout_cap = c;
+  }
+  if (!slot->stream) HIPCHK(hipStreamCreateWithFlags(&slot->stream, hipStreamNonBlocking));
+  if (!slot->ev) HIPCHK(hipEventCreateWithFlags(&slot->ev, hipEventDisableTiming));
+  if (!recs.empty())
+    HIPCHK(hipMemcpyAsync(slot->recs, recs.data(), recs.size() * sizeof(BRec), hipMemcpyHostToDevice, slot->stream));
+  HIPCHK(hipMemcpyAsync(slot->insts, insts.data(), insts.size() * sizeof(BInst), hipMemcpyHostToDevice, slot->stream));
+  HIPCHK(hipMemcpyAsync(slot->order, order.data(), order.size() * sizeof(int32_t), hipMemcpyHostToDevice, slot->stream));
+  HIPCHK(hipMemcpyAsync(slot->jobs, jobs.data(), jobs.size() * sizeof(SampleJob), hipMemcpyHostToDevice, slot->stream));
+  const size_t lds = ((size_t)1 << maxn) * B;
+  if (setup_lds_batch(prec, lds) != 0) return fail(QSV_E_DEVICE, "serving: LDS reservation failed");
+  launch_lds_batch(prec, (unsigned)batch.size(), lds, slot->insts, slot->recs, nullptr, nullptr, slot->order, nullptr,
+                   slot->arena, slot->stream);
+  launch_batch_sample(prec, slot->arena, slot->jobs, (unsigned)jobs.size(), slot->d_out, slot->stream);
+  if (nout)
+    HIPCHK(hipMemcpyAsync(slot->h_out, slot->d_out, nout * sizeof(uint64_t), hipMemcpyDeviceToHost, slot->stream));
+  HIPCHK(hipEventRecord(slot->ev, slot->stream));
+  HIPCHK(hipGetLastError());
+  slot->reqs = batch;
+  slot->busy = true;
+  sv->stats.small_batches++;
+  sv->stats.small_batched_requests += batch.size();
+  return QSV_OK;
+}
+
+int launch_large(qsv_server* sv, SrvReq* r)
+{
+  int rc = take_state(sv, r->n, &r->st);
+  if (rc) return rc;
+  qsv_state* s = r->st;
+  launch_fill_basis(s->prec, s->amps, s->S, 0, s->stream);
+  if (sv->cfg.flags & QSV_FLAG_FUSE) rc = run_fused(s, r->gates);
+  else {
+    for (const auto& g : r->gates)
+      if ((rc = run_direct(s, g))) break;
+  }
+  if (rc) return rc;
+  if (!s->prefix) HIPCHK(hipMalloc(&s->prefix, std::max<uint64_t>(1, s->S >> kSampleChunkLog) * sizeof(double)));
+  if (r->shots) {
+    HIPCHK(hipMalloc(&r->d_out, r->shots * sizeof(uint64_t)));
+    HIPCHK(hipHostMalloc((void**)&r->h_out, r->shots * sizeof(uint64_t), hipHostMallocDefault));
+  }
+  launch_sample(s->prec, s->amps, s->S, s->prefix, r->shots, r->seed, r->d_out, s->stream);
+  if (r->shots)
+    HIPCHK(hipMemcpyAsync(r->h_out, r->d_out, r->shots * sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipEventCreateWithFlags(&r->ev, hipEventDisableTiming));
+  HIPCHK(hipEventRecord(r->ev, s->stream));
+  HIPCHK(hipGetLastError());
+  return QSV_OK;
+}
+
+void release_large(qsv_server* sv, SrvReq* r)
+{
+  if (r->ev) hipEventDestroy(r->ev);
+  if (r->d_out) hipFree(r->d_out);
+  if (r->h_out) hipHostFree(r->h_out);
+  r->ev = nullptr;
+  r->d_out = nullptr;
+  r->h_out = nullptr;
+  if (r->st) give_state(sv, r->st);
+  r->st = nullptr;
+}
+
+void server_loop(qsv_server* sv)
+{
+  hipSetDevice(g_device);
+  std::unique_lock<std::mutex> lk(sv->mu);
+  while (true) {
+    bool progressed = false;
+    // 1. admission: arrival order, while the states fit the budget
+    const int cap = sv->cfg.max_inflight > 0 ? sv->cfg.max_inflight : 32;
+    while (!sv->queue.empty()) {
+      SrvReq* r = sv->queue.front();
+      if (sv->in_use + r->bytes > sv->budget && sv->in_use > 0) break;
+      if (sv->inflight() >= cap) break;
+      sv->queue.pop_front();
+      sv->in_use += r->bytes;
+      r->t_admit = Clock::now();
+      r->status = QSV_REQ_RUNNING;
+      (r->small ? sv->pending_small : sv->pending_large).push_back(r);
+      progressed = true;
+    }
+    // 2. launches (outside the lock: submitters keep queueing meanwhile)
+    std::vector<SrvReq*> smalls, larges;
+    if (!sv->pending_small.empty()) {
+      bool free_slot = false;
+      for (auto& s : sv->slots) free_slot |= !s.busy;
+      if (free_slot) smalls.swap(sv->pending_small);
+    }
+    larges.swap(sv->pending_large);
+    if (!smalls.empty() || !larges.empty()) {
+      lk.unlock();
+      std::vector<std::pair<SrvReq*, std::string>> failed;
+      if (!smalls.empty()) {
+        const int rc = launch_small(sv, smalls);
+        if (rc) for (SrvReq* r : smalls) failed.push_back({r, "batched launch failed: " + g_err});
+      }
+      std::vector<SrvReq*> started;
+      for (SrvReq* r : larges) {
+        const int rc = launch_large(sv, r);
+        if (rc) failed.push_back({r, g_err});
+        else started.push_back(r);
+      }
+      lk.lock();
+      for (SrvReq* r : started) sv->running_large.push_back(r);
+      for (auto& f : failed) {
+        release_large(sv, f.first);
+        finish(sv, f.first, QSV_REQ_FAILED, f.second);
+      }
+      if (!failed.empty()) sv->cv_done.notify_all();
+      progressed = true;
+    }
+    // 3. completions
+    bool done_any = false;
+    for (size_t k = 0; k < sv->running_large.size();) {
+      SrvReq* r = sv->running_large[k];
+      if (hipEventQuery(r->ev) == hipSuccess) {
+        r->outcomes.assign(r->h_out, r->h_out + r->shots);
+        release_large(sv, r);
+        finish(sv, r, QSV_REQ_COMPLETED, "");
+        sv->running_large[k] = sv->running_large.back();
+        sv->running_large.pop_back();
+        done_any = true;
+      } else {
+        ++k;
+      }
+    }
+    for (auto& slot : sv->slots) {
+      if (slot.busy && hipEventQuery(slot.ev) == hipSuccess) {
+        for (size_t k = 0; k < slot.reqs.size(); ++k) {
+          SrvReq* r = slot.reqs[k];
+          r->outcomes.assign(slot.h_out + slot.out_offsets[k], slot.h_out + slot.out_offsets[k] + r->shots);
+          finish(sv, r, QSV_REQ_COMPLETED, "");
+        }
+        slot.reqs.clear();
+        slot.busy = false;
+        done_any = true;
+      }
+    }
+    if (done_any) {
+      sv->cv_done.notify_all();
+      progressed = true;
+    }
+    const bool busy = !sv->running_large.empty() ||
+                      std::any_of(std::begin(sv->slots), std::end(sv->slots), [](const SmallSlot& s) { return s.busy; });
+    if (sv->stop && sv->queue.empty() && !busy && sv->pending_small.empty() && sv->pending_large.empty()) break;
+    if (!progressed) {
+      if (busy) sv->cv_work.wait_for(lk, std::chrono::microseconds(50));
+      else sv->cv_work.wait_for(lk, std::chrono::milliseconds(5));
+    }
+  }
+}
+
+void fill_info(const SrvReq* r, qsv_request_info* info)
+{
+  if (!info) return;
+  std::memset(info, 0, sizeof(*info));
+  info->status = r->status;
+  info->num_qubits = r->n;
+  info->shots = r->shots;
+  auto ms = [](Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+  };
+  if (r->status != QSV_REQ_QUEUED) info->queue_ms = ms(r->t_submit, r->t_admit);
+  if (r->status >= QSV_REQ_COMPLETED) info->latency_ms = ms(r->t_submit, r->t_done);
+  std::snprintf(info->error, sizeof(info->error), "%s", r->err.c_str());
+}
+
+}  // namespace
+
+extern "C" {
+
+int qsv_server_create(const qsv_server_config* cfg, qsv_server** out)
+{
+  if (!cfg || !out) return fail(QSV_E_INVALID, "null argument");
+  if (cfg->precision != QSV_C64 && cfg->precision != QSV_C128) return fail(QSV_E_INVALID, "unknown precision");
+  int rc = ensure_init();
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(g_device));
+  auto* sv = new qsv_server();
+  sv->cfg = *cfg;
+  sv->lds_n = cfg->precision == QSV_C128 ? 13 : 14;
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+    delete sv;
+    return fail(QSV_E_DEVICE, "hipMemGetInfo failed");
+  }
+  sv->budget = cfg->hbm_budget_bytes ? cfg->hbm_budget_bytes : (uint64_t)(0.8 * (double)fr);
+  sv->stats.hbm_budget_bytes = sv->budget;
+  sv->th = std::thread(server_loop, sv);
+  *out = sv;
+  return QSV_OK;
+}
+
+int qsv_server_submit(qsv_server* sv, int num_qubits, const qsv_op* ops, uint64_t n_ops, uint64_t shots,
+                      uint64_t seed, uint64_t* request_id)
+{
+  if (!sv || !request_id || (n_ops && !ops)) return fail(QSV_E_INVALID, "null argument");
+  if (num_qubits < 1 || num_qubits > 34) return fail(QSV_E_INVALID, "num_qubits must be in [1, 34]");
+  auto* r = new SrvReq();
+  r->t_submit = Clock::now();
+  r->n = num_qubits;
+  r->shots = shots;
+  r->seed = seed;
+  uint64_t mk = 0;
+  for (uint64_t i = 0; i < n_ops; ++i) {
+    if (ops[i].kind == QSV_OP_MEASURE) {
+      delete r;
+      return fail(QSV_E_UNSUPPORTED, "op " + std::to_string(i) + ": mid-circuit measurement is not served");
+    }
+    const int rc = decode_op(num_qubits, ops[i], sv->cfg.flags, seed, &mk, r->gates);
+    if (rc) {
+      delete r;
+      g_err = "op " + std::to_string(i) + ": " + g_err;
+      return rc;
+    }
+  }
+  const uint64_t B = sv->cfg.precision == QSV_C128 ? 16 : 8;
+  r->bytes = (1ull << num_qubits) * B;
+  r->small = sv->cfg.batch_small && num_qubits <= sv->lds_n;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (r->bytes > sv->budget) {
+      delete r;
+      return fail(QSV_E_NOMEM, "request state exceeds the HBM budget");
+    }
+    r->id = sv->next_id++;
+    sv->reqs[r->id] = r;
+    sv->queue.push_back(r);
+    sv->stats.submitted++;
+    sv->stats.max_queue_depth = std::max<uint64_t>(sv->stats.max_queue_depth, sv->queue.size());
+    *request_id = r->id;
+  }
+  sv->cv_work.notify_one();
+  return QSV_OK;
+}
+
+int qsv_server_status(qsv_server* sv, uint64_t request_id, qsv_request_info* info)
+{
+  if (!sv) return fail(QSV_E_INVALID, "null server");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  auto it = sv->reqs.find(request_id);
+  if (it == sv->reqs.end()) return fail(QSV_E_NOTFOUND, "unknown request id");
+  fill_info(it->second, info);
+  return QSV_OK;
+}
+
+int qsv_server_wait(qsv_server* sv, uint64_t request_id, double timeout_s, qsv_request_info* info,
+                    uint64_t* outcomes)
+{
+  if (!sv) return fail(QSV_E_INVALID, "null server");
+  std::unique_lock<std::mutex> lk(sv->mu);
+  auto it = sv->reqs.find(request_id);
+  if (it == sv->reqs.end()) return fail(QSV_E_NOTFOUND, "unknown request id");
+  SrvReq* r = it->second;
+  const auto deadline = Clock::now() + std::chrono::duration<double>(timeout_s);
+  while (r->status < QSV_REQ_COMPLETED) {
+    if (sv->cv_done.wait_until(lk, deadline) == std::cv_status::timeout && r->status < QSV_REQ_COMPLETED) {
+      fill_info(r, info);
+      return fail(QSV_E_STATE, "request not complete before the timeout");
+    }
+  }
+  fill_info(r, info);
+  if (outcomes && r->status == QSV_REQ_COMPLETED) std::memcpy(outcomes, r->outcomes.data(), r->shots * sizeof(uint64_t));
+  sv->reqs.erase(it);
+  delete r;
+  return QSV_OK;
+}
+
+int qsv_server_stats_read(qsv_server* sv, qsv_server_stats* out)
+{
+  if (!sv || !out) return fail(QSV_E_INVALID, "null argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  *out = sv->stats;
+  out->queued_now = sv->queue.size();
+  out->inflight_now = (uint64_t)sv->inflight();
+  out->hbm_in_use_bytes = sv->in_use;
+  out->hbm_budget_bytes = sv->budget;
+  return QSV_OK;
+}
+
+int qsv_server_destroy(qsv_server* sv)
+{
+  if (!sv) return QSV_OK;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->stop = true;
+  }
+  sv->cv_work.notify_all();
+  if (sv->th.joinable()) sv->th.join();
+  for (auto& kv : sv->reqs) delete kv.second;
+  for (auto& fl : sv->free_states)
+    for (qsv_state* s : fl) qsv_state_destroy(s);
+  for (auto& s : sv->slots) {
+    if (s.stream) hipStreamSynchronize(s.stream);
+    hipFree(s.arena);
+    hipFree(s.recs);
+    hipFree(s.insts);
+    hipFree(s.order);
+    hipFree(s.jobs);
+    hipFree(s.d_out);
+    if (s.h_out) hipHostFree(s.h_out);
+    if (s.ev) hipEventDestroy(s.ev);
+    if (s.stream) hipStreamDestroy(s.stream);
+  }
+  delete sv;
   return QSV_OK;
 }
 

@@ -89,6 +89,78 @@ void launch_outcome_hist(int prec, const void* psi, uint64_t S, int m, const int
 void launch_collapse(int prec, void* psi, uint64_t S, int m, const int* pos, uint64_t sel, double scale,
                      hipStream_t st);
 
-// Fused LDS-tiled pass (qsv_fused.hip).
-struct TileGate;  // defined in qsv_fused.hip
+// ---- fused tiled passes (qsv_fuse.hip) ---------------------------------------
+constexpr int kTileBitsC128 = 12;   // 4096 x 16 B = 64 KiB of LDS per tile
+constexpr int kTileBitsC64 = 13;    // 8192 x  8 B = 64 KiB
+constexpr int kMaxPassGates = 96;
+constexpr int kMaxPassStages = 32;
+
+enum { FG_DENSE = 0, FG_PHASE = 1, FG_SWAP = 2, FG_BLOCKADE = 3 };
+
+// A gate in tile-local bit coordinates.
+struct FGate {
+  int32_t kind;        // FG_*
+  int32_t a, b;        // dense/phase target a; swap bits a, b
+  uint32_t cmask;      // local control bits (all must be 1)
+  uint32_t bmask;      // blockade bits
+  int32_t phase_only;  // FG_PHASE with d0 == 1
+  double m[8];         // dense: row-major 2x2; phase: d0 = m[0..1], d1 = m[6..7]; swap/blockade: phase m[0..1]
+};
+
+struct FStage {
+  uint32_t regmask;    // tile-local bits held in registers during this stage
+  int32_t g0, g1;      // gates [g0, g1)
+};
+
+struct FPass {
+  int32_t nstages;
+  int32_t ngates;
+  uint32_t load_mask, store_mask;   // register masks used for the global load / store
+  int32_t pos[16];                  // global index bit of tile-local bit i
+  int32_t sorted_pos[16];           // the same bits in ascending order
+  FStage stages[kMaxPassStages];
+  FGate gates[kMaxPassGates];
+};
+
+void launch_tile(int prec, void* psi, const FPass* dpass, uint64_t ntiles, hipStream_t st);
+
+// ---- batched small circuits (qsv_batch.hip) -----------------------------------
+constexpr int kBatchThreads = 256;
+
+// One decoded gate of a batched circuit, in index-bit coordinates.
+struct BRec {
+  int32_t kind;        // FG_DENSE, FG_PHASE, FG_SWAP or FG_BLOCKADE
+  int32_t a, b;        // target bit / swap bits
+  int32_t phase_only;
+  uint64_t cmask;      // control bits (all must be 1)
+  uint64_t bmask;      // blockade bits
+  double m[8];         // as FGate
+};
+
+struct BInst {
+  int32_t n;
+  int32_t pad;
+  uint64_t g0, g1;       // gate records [g0, g1)
+  uint64_t t0, t1;       // Z terms [t0, t1)
+  uint64_t out_offset;   // amplitude offset of the final state in the output arena, ~0 for none
+};
+
+int setup_lds_batch(int prec, size_t lds_bytes);
+void launch_lds_batch(int prec, unsigned nblocks, size_t lds_bytes, const BInst* inst, const BRec* gates,
+                      const uint64_t* tz, const double* tc, const int32_t* order, double* energies,
+                      void* out_states, hipStream_t st);
+void launch_zexpect(int prec, const void* psi, uint64_t S, const uint64_t* tz, const double* tc, uint64_t nterms,
+                    double* partials, double* out, hipStream_t st);
+
+// Shots for many small states stored back to back in an arena.
+struct SampleJob {
+  uint64_t state_offset;   // amplitude offset in the arena
+  uint64_t out_offset;     // outcome offset
+  uint64_t shots;
+  uint64_t seed;
+  int32_t n;
+  int32_t pad;
+};
+void launch_batch_sample(int prec, const void* arena, const void* jobs, unsigned njobs, uint64_t* outcomes,
+                         hipStream_t st);
 }  // namespace qsv

@@ -184,6 +184,99 @@ class DeviceState:
                 for i, name in enumerate(_abi.KCLASS_NAMES)}
 
 
+def batch_expect_z(instances, precision="c64") -> np.ndarray:
+    """Run many independent circuits from |0..0> and return each one's
+    <H> = sum_k c_k <Z-string_k> (qsv_batch_expect_z).
+
+    instances: iterable of (num_qubits, ops (qsv_op records), terms) where terms
+    is a list of (coefficient, pauli_string) with only I/Z characters.
+    """
+    init()
+    nq, opoff, toff, zm, co, all_ops = [], [0], [0], [], [], []
+    for (n, ops, terms) in instances:
+        nq.append(int(n))
+        ops = np.ascontiguousarray(ops, dtype=_abi.OP_DTYPE)
+        all_ops.append(ops)
+        opoff.append(opoff[-1] + len(ops))
+        for (c, s) in terms:
+            if len(s) != n or set(s) - {"I", "Z"}:
+                raise ValueError(f"batched Hamiltonians take I/Z strings of length {n}: {s!r}")
+            zm.append(sum(1 << q for q, ch in enumerate(s) if ch == "Z"))
+            co.append(float(c))
+        toff.append(len(zm))
+    N = len(nq)
+    nq_a = np.asarray(nq, dtype=np.int32)
+    opoff_a = np.asarray(opoff, dtype=np.uint64)
+    toff_a = np.asarray(toff, dtype=np.uint64)
+    zm_a = np.asarray(zm if zm else [0], dtype=np.uint64)
+    co_a = np.asarray(co if co else [0.0], dtype=np.float64)
+    ops_a = np.ascontiguousarray(np.concatenate(all_ops) if all_ops else np.zeros(1, _abi.OP_DTYPE))
+    desc = _abi.QsvBatchDesc(N, nq_a.ctypes.data, opoff_a.ctypes.data, ops_a.ctypes.data, toff_a.ctypes.data,
+                             zm_a.ctypes.data, co_a.ctypes.data, PRECISIONS[precision])
+    out = np.zeros(N, dtype=np.float64)
+    check(lib().qsv_batch_expect_z(ctypes.byref(desc), _ptr(out)))
+    return out
+
+
+class Server:
+    """The native request loop (qsv_server_*): continuous batching with
+    admission by state bytes against the HBM budget."""
+
+    def __init__(self, precision="c64", hbm_budget_bytes: int = 0, max_inflight: int = 0,
+                 flags: int = _abi.QSV_FLAG_FUSE, batch_small: bool = True):
+        init()
+        cfg = _abi.QsvServerConfig(int(hbm_budget_bytes), PRECISIONS[precision], int(max_inflight),
+                                   int(flags), 1 if batch_small else 0)
+        h = ctypes.c_void_p()
+        check(lib().qsv_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+
+    def submit(self, num_qubits: int, ops: np.ndarray, shots: int, seed: int) -> int:
+        ops = np.ascontiguousarray(ops, dtype=_abi.OP_DTYPE)
+        rid = ctypes.c_uint64()
+        check(lib().qsv_server_submit(self._h, int(num_qubits), _ptr(ops), len(ops), int(shots),
+                                      seed & (2**64 - 1), ctypes.byref(rid)))
+        return rid.value
+
+    @staticmethod
+    def _info(info) -> dict:
+        return {"status": _abi.REQ_STATUS[info.status], "num-qubits": info.num_qubits, "shots": info.shots,
+                "queue-ms": info.queue_ms, "latency-ms": info.latency_ms, "error": info.error.decode()}
+
+    def status(self, rid: int) -> dict:
+        info = _abi.QsvRequestInfo()
+        check(lib().qsv_server_status(self._h, rid, ctypes.byref(info)))
+        return self._info(info)
+
+    def wait(self, rid: int, shots: int, timeout_s: float = 600.0):
+        info = _abi.QsvRequestInfo()
+        out = np.zeros(max(1, int(shots)), dtype=np.uint64)
+        check(lib().qsv_server_wait(self._h, rid, float(timeout_s), ctypes.byref(info), _ptr(out)))
+        return self._info(info), out[:int(shots)]
+
+    def stats(self) -> dict:
+        st = _abi.QsvServerStats()
+        check(lib().qsv_server_stats_read(self._h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in _abi.QsvServerStats._fields_}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None:
+            lib().qsv_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def uniform(seed: int, k: int) -> float:
     """Counter-based uniform shared with the device sampler (qsv_uniform)."""
     return lib().qsv_uniform(seed & (2**64 - 1), k)

@@ -1,0 +1,80 @@
+"""Batched circuits in one launch (qsv_batch_expect_z): per-instance <H_C> of
+QAOA MaxCut instances against the FP64 oracle, and against the single-state
+path."""
+import sys
+
+import numpy as np
+import pytest
+
+from qclojure_amd import circuit as qc
+from qclojure_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    from qclojure_amd import engine
+    engine.init(0)
+    return engine
+
+
+def _oracle_energy(oracle, inst):
+    n = inst["n"]
+    psi = oracle.run(n, qc.decode_circuit(inst["circuit"]))
+    e = None
+    for t in inst["hamiltonian"]:
+        v = t["coefficient"] * oracle.pauli_expectation(psi, n, t["pauli-string"])
+        e = v if e is None else e + v
+    return e
+
+
+def _batch(insts):
+    return [(i["n"], qc.decode_circuit(i["circuit"]), [(t["coefficient"], t["pauli-string"]) for t in i["hamiltonian"]])
+            for i in insts]
+
+
+@pytest.mark.parametrize("prec,tol", [("c128", 1e-10), ("c64", 1e-5)])
+def test_batch_qaoa_vs_oracle(eng, oracle, prec, tol):
+    insts = [wl.qaoa_instance(1000 + k) for k in range(36)]
+    # make sure every size class is present, including the LDS-resident and pooled paths
+    insts += [wl.qaoa_instance(77, n_choices=(n,)) for n in (8, 12, 14, 16, 20)]
+    print(f"sizes {[i['n'] for i in insts]}", file=sys.stderr, flush=True)
+    got = eng.batch_expect_z(_batch(insts), prec)
+    for inst, g in zip(insts, got):
+        exp = _oracle_energy(oracle, inst)
+        assert abs(g - exp) < tol * max(1.0, abs(exp)), (inst["n"], inst["p"], g, exp)
+
+
+def test_batch_matches_single_state_path(eng):
+    from qclojure_amd.hamiltonian import hamiltonian_expectation
+    insts = [wl.qaoa_instance(2000 + k, n_choices=(10, 16)) for k in range(6)]
+    got = eng.batch_expect_z(_batch(insts), "c128")
+    for inst, g in zip(insts, got):
+        with eng.DeviceState(inst["n"], "c128") as s:
+            s.apply(qc.decode_circuit(inst["circuit"]))
+            e = hamiltonian_expectation(inst["hamiltonian"], s)
+        assert abs(g - e) < 1e-10
+
+
+def test_batch_general_gates(eng, oracle):
+    """Every gate kind through the batched interpreter (small and pooled sizes)."""
+    from test_gpu_parity import _random_program
+    rng = np.random.default_rng(5)
+    insts = []
+    for n in (3, 6, 9, 13, 15):
+        ops = _random_program(rng, n, 80)
+        terms = [(float(rng.normal()), "".join(rng.choice(["I", "Z"], size=n))) for _ in range(5)]
+        insts.append((n, ops, terms))
+    got = eng.batch_expect_z(insts, "c128")
+    for (n, ops, terms), g in zip(insts, got):
+        psi = oracle.run(n, ops)
+        exp = sum(c * oracle.pauli_expectation(psi, n, s) for c, s in terms)
+        assert abs(g - exp) < 1e-10, n
+
+
+def test_batch_rejects_measurements(eng):
+    from qclojure_amd._abi import OP, QsvError
+    ops = qc.ops_from_tuples([(OP["H"], 0, 0, 0, 0, 0.0), (OP["MEASURE"], 0, 0, 0, 1, 0.0)])
+    with pytest.raises(QsvError, match="not batched"):
+        eng.batch_expect_z([(2, ops, [(1.0, "ZI")])], "c64")

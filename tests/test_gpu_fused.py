@@ -1,0 +1,120 @@
+"""Parity of the fused tiled-pass path (QSV_FLAG_FUSE) with the oracle and with
+the per-gate path."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from qclojure_amd import circuit as qc
+from qclojure_amd import workloads as wl
+from qclojure_amd._abi import OP, QSV_FLAG_FUSE, QSV_FLAG_MSB_SWAP
+
+pytestmark = pytest.mark.gpu
+
+
+def note(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def eng():
+    from qclojure_amd import engine
+    engine.init(0)
+    return engine
+
+
+def _program(rng, n, count):
+    kinds1 = ["X", "Y", "Z", "H", "S", "SDG", "T", "TDG", "RX", "RY", "RZ", "PHASE"]
+    kinds2 = ["CNOT", "CZ", "CY", "CRX", "CRY", "CRZ", "RYDBERG_CZ", "RYDBERG_CPHASE", "SWAP", "ISWAP"]
+    rows = []
+    for _ in range(count):
+        r = rng.uniform()
+        ang = float(rng.uniform(0, 2 * math.pi))
+        if r < 0.5:
+            rows.append((OP[rng.choice(kinds1)], int(rng.integers(n)), 0, 0, 0, ang))
+        elif r < 0.85:
+            a, b = rng.choice(n, 2, replace=False)
+            rows.append((OP[rng.choice(kinds2)], int(a), int(b), 0, 0, ang))
+        elif r < 0.95:
+            a, b, c = rng.choice(n, 3, replace=False)
+            rows.append((OP[rng.choice(["TOFFOLI", "FREDKIN"])], int(a), int(b), int(c), 0, 0.0))
+        else:
+            qs = rng.choice(n, 3, replace=False)
+            rows.append((OP["RYDBERG_BLOCKADE"], 0, 0, 0, int(sum(1 << int(q) for q in qs)), ang))
+    return qc.ops_from_tuples(rows)
+
+
+def _run(eng, n, ops, prec="c128", flags=0, seed=0, init=None):
+    with eng.DeviceState(n, prec) as s:
+        if init is not None:
+            s.upload(init)
+        s.apply(ops, flags, seed)
+        return s.download()
+
+
+@pytest.mark.parametrize("n", [12, 13, 14, 16])
+def test_fused_random_programs_c128(eng, oracle, n):
+    rng = np.random.default_rng(500 + n)
+    init = oracle.run(n, _program(rng, n, 20))
+    ops = _program(rng, n, 300)
+    note(f"fused c128 n={n}")
+    exp = oracle.apply_program(init, n, ops)
+    got = _run(eng, n, ops, flags=QSV_FLAG_FUSE, init=init)
+    assert np.max(np.abs(got - exp)) < 1e-12
+    got = _run(eng, n, ops, flags=QSV_FLAG_FUSE | QSV_FLAG_MSB_SWAP, init=init)
+    assert np.max(np.abs(got - oracle.apply_program(init, n, ops, flags=QSV_FLAG_MSB_SWAP))) < 1e-12
+
+
+@pytest.mark.parametrize("n", [13, 15, 17])
+def test_fused_random_programs_c64(eng, oracle, n):
+    rng = np.random.default_rng(900 + n)
+    init = oracle.run(n, _program(rng, n, 20))
+    ops = _program(rng, n, 300)
+    note(f"fused c64 n={n}")
+    exp = oracle.apply_program(init, n, ops)
+    got = _run(eng, n, ops, prec="c64", flags=QSV_FLAG_FUSE, init=init)
+    assert abs(np.vdot(exp, got)) ** 2 > 1 - 1e-9
+    assert np.max(np.abs(got - exp)) < 1e-6
+
+
+def test_fused_every_position(eng, oracle):
+    """Dense gates on every bit, controls on every bit, swaps on low/high mixes."""
+    n = 16
+    rng = np.random.default_rng(3)
+    rows = []
+    for q in range(n):
+        rows += [(OP["RY"], q, 0, 0, 0, 0.3 + q), (OP["H"], q, 0, 0, 0, 0.0), (OP["RZ"], q, 0, 0, 0, 0.1 * q)]
+        c = (q + 5) % n
+        rows += [(OP["CRX"], c, q, 0, 0, 0.2 + q), (OP["CNOT"], q, c, 0, 0, 0.0), (OP["T"], q, 0, 0, 0, 0.0)]
+        rows += [(OP["SWAP"], q, (q + 1) % n, 0, 0, 0.0), (OP["ISWAP"], q, (q + 9) % n, 0, 0, 0.0)]
+    ops = qc.ops_from_tuples(rows)
+    init = oracle.run(n, _program(rng, n, 30))
+    exp = oracle.apply_program(init, n, ops)
+    assert np.max(np.abs(_run(eng, n, ops, flags=QSV_FLAG_FUSE, init=init) - exp)) < 1e-12
+
+
+def test_fused_measure_barrier(eng, oracle):
+    n = 14
+    rows = [(OP["H"], q, 0, 0, 0, 0.0) for q in range(n)] + [(OP["MEASURE"], 0, 0, 0, 0b1011, 0.0)]
+    rows += [(OP["RY"], q, 0, 0, 0, 0.3 * q) for q in range(n)] + [(OP["CNOT"], 0, 13, 0, 0, 0.0)]
+    rows += [(OP["MEASURE"], 0, 0, 0, 1 << 13, 0.0)] + [(OP["RX"], q, 0, 0, 0, 0.2) for q in range(n)]
+    ops = qc.ops_from_tuples(rows)
+    exp = oracle.run(n, ops, seed=77)
+    assert np.max(np.abs(_run(eng, n, ops, flags=QSV_FLAG_FUSE, seed=77) - exp)) < 1e-12
+
+
+def test_fused_equals_per_gate_layers(eng):
+    """The 22-qubit random-layer circuit: fused and per-gate paths agree."""
+    n = 22
+    ops = wl.random_layer_ops(n, 10, seed=4)
+    a = _run(eng, n, ops)
+    b = _run(eng, n, ops, flags=QSV_FLAG_FUSE)
+    assert np.max(np.abs(a - b)) < 1e-12
+
+
+def test_small_states_fall_back(eng, oracle):
+    n = 8
+    rng = np.random.default_rng(1)
+    ops = _program(rng, n, 100)
+    assert np.max(np.abs(_run(eng, n, ops, flags=QSV_FLAG_FUSE) - oracle.run(n, ops))) < 1e-12
