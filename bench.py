@@ -100,6 +100,168 @@ def position_sweep(engine, n: int, prec: str, reps: int = 3) -> dict:
     return table
 
 
+def _touched(ops: "np.ndarray", n: int) -> float:
+    """Amplitude updates of a program: S per 1-qubit gate, S/2 per controlled gate."""
+    import numpy as np
+    from qclojure_amd._abi import OP
+    k = ops["kind"]
+    ctl = np.isin(k, [OP["CNOT"], OP["CZ"], OP["CRX"], OP["CRY"], OP["CRZ"], OP["CY"]])
+    return float((len(ops) - ctl.sum()) * (1 << n) + ctl.sum() * (1 << (n - 1)))
+
+
+def bench_batched_qaoa(engine, count: int, seed: int, plain_every: int) -> dict:
+    """4096 MaxCut QAOA instances (n in {8..20} even, p in 1..4), complex64:
+    per-instance <H_C> from one batched call vs the plain path (one circuit at a
+    time: per-gate kernels, then a Pauli-expectation pass per 8 terms)."""
+    from qclojure_amd import workloads as wl
+    insts = [wl.qaoa_instance(seed * 100003 + k) for k in range(count)]
+    batch = []
+    upd = 0.0
+    for inst in insts:
+        ops = wl.qaoa_ops(inst)
+        batch.append((inst["n"], ops, [(t["coefficient"], t["pauli-string"]) for t in inst["hamiltonian"]]))
+        upd += _touched(ops, inst["n"])
+    engine.batch_expect_z(batch[:64], "c64")               # warm-up: kernels, pooled states
+    t0 = time.perf_counter()
+    e64 = engine.batch_expect_z(batch, "c64")
+    t_batch = time.perf_counter() - t0
+    e128 = engine.batch_expect_z(batch, "c128")
+    err = float(max(abs(a - b) for a, b in zip(e64, e128)))
+    # plain path on every plain_every-th instance
+    sample = batch[::plain_every]
+    s_upd = sum(_touched(ops, n) for (n, ops, _) in sample)
+    states = {}
+    t0 = time.perf_counter()
+    e_plain = []
+    for (n, ops, terms) in sample:
+        if n not in states:
+            states[n] = engine.DeviceState(n, "c64")
+        s = states[n]
+        s.init_basis(0)
+        s.apply(ops)
+        vals = s.expect_pauli([p for _, p in terms])
+        e_plain.append(sum(c * v for (c, _), v in zip(terms, vals)))
+    t_plain = time.perf_counter() - t0
+    for s in states.values():
+        s.destroy()
+    plain_err = float(max(abs(a - b) for a, b in zip(e_plain, e64[::plain_every])))
+    sizes = sorted({b[0] for b in batch})
+    return {"instances": count, "qubits": sizes, "gates": int(sum(len(b[1]) for b in batch)),
+            "batched_s": round(t_batch, 4), "batched_amp_updates_per_s": round(upd / t_batch, 1),
+            "plain_sample_instances": len(sample), "plain_s": round(t_plain, 4),
+            "plain_amp_updates_per_s": round(s_upd / t_plain, 1),
+            "speedup": round((upd / t_batch) / (s_upd / t_plain), 2),
+            "max_abs_energy_err_c64_vs_c128": err, "max_abs_energy_diff_plain_vs_batched": plain_err,
+            "note": "plain = one circuit at a time, per-gate kernels + Pauli reductions; timed on a stratified sample"}
+
+
+def bench_serving(engine, rates, n_req: int, n_plain: int, seed: int) -> dict:
+    """Poisson request stream (n log-uniform in [10, 30], depth 10..100 layers,
+    1000 shots, complex64): the native serving loop vs the plain sequential path.
+    The request contents are fixed; each rate rescales the same unit-rate
+    arrival times. Plain latencies come from measured per-request service
+    times through a single FIFO server."""
+    import threading
+    import numpy as np
+    from qclojure_amd import workloads as wl
+    from qclojure_amd._abi import QSV_FLAG_FUSE
+    stream = wl.serving_stream(n_req, 1.0, seed)
+    reqs = [(r["n"], wl.random_layer_ops(r["n"], r["depth"], r["seed"]), r["shots"], r["seed"]) for r in stream]
+    unit_arrivals = [r["arrival-s"] for r in stream]
+    gates = sum(len(r[1]) for r in reqs)
+    # plain: one request at a time, per-gate kernels, then shots
+    service = []
+    states = {}
+    for (n, ops, shots, sd) in reqs[:n_plain]:
+        t0 = time.perf_counter()
+        if n not in states:
+            states[n] = engine.DeviceState(n, "c64")
+        s = states[n]
+        s.init_basis(0)
+        s.apply(ops)
+        s.sample(shots, sd)
+        service.append(time.perf_counter() - t0)
+    for s in states.values():
+        s.destroy()
+    out = {"requests": n_req, "gates": gates, "plain_requests": n_plain, "rates": []}
+    for lam in rates:
+        arr = [a / lam for a in unit_arrivals]
+        # plain FIFO queue over the measured service times
+        done, lat = 0.0, []
+        for a, sv in zip(arr[:n_plain], service):
+            done = max(a, done) + sv
+            lat.append(done - a)
+        plain = {"circuits_per_s": round(n_plain / (done - arr[0]), 2),
+                 "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 1),
+                 "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 1)}
+        # served: real arrivals into the native loop
+        with engine.Server("c64", flags=QSV_FLAG_FUSE) as srv:
+            ids = [None] * n_req
+            t_start = time.perf_counter()
+
+            def submitter():
+                for k, (n, ops, shots, sd) in enumerate(reqs):
+                    delay = arr[k] - (time.perf_counter() - t_start)
+                    if delay > 0:
+                        time.sleep(delay)
+                    ids[k] = srv.submit(n, ops, shots, sd)
+
+            th = threading.Thread(target=submitter)
+            th.start()
+            lats = []
+            for k in range(n_req):
+                while ids[k] is None:
+                    time.sleep(0.0005)
+                info, _ = srv.wait(ids[k], reqs[k][2])
+                lats.append(info["latency-ms"])
+            th.join()
+            t_total = time.perf_counter() - t_start
+            st = srv.stats()
+        served = {"circuits_per_s": round(n_req / t_total, 2),
+                  "p50_ms": round(float(np.percentile(lats, 50)), 1),
+                  "p99_ms": round(float(np.percentile(lats, 99)), 1),
+                  "small_batched": int(st["small_batched_requests"]), "batches": int(st["small_batches"]),
+                  "max_queue_depth": int(st["max_queue_depth"])}
+        out["rates"].append({"lambda_per_s": lam, "served": served, "plain": plain,
+                             "throughput_ratio": round(served["circuits_per_s"] / plain["circuits_per_s"], 2)})
+        log(f"bench: serving lambda={lam}: served {served} plain {plain}")
+    return out
+
+
+def bench_precision(engine, n: int, layers: int, seed: int) -> dict:
+    """Seeded random product state, the reference QFT, then layers of random U3
+    (as RZ RY RZ) and CZ on a random matching: complex64 vs complex128, both
+    through fused passes, compared every 10 layers."""
+    from qclojure_amd import algorithms as alg
+    from qclojure_amd import circuit as qc
+    from qclojure_amd import workloads as wl
+    from qclojure_amd._abi import QSV_FLAG_FUSE
+    circ = wl.precision_circuit(n, layers, seed)
+    ops = qc.decode_circuit(circ)
+    prologue = 2 * n + len(alg.quantum_fourier_transform_circuit(n)["operations"])
+    per_layer = 3 * n + n // 2
+    a = engine.DeviceState(n, "c128")
+    b = engine.DeviceState(n, "c64")
+    rows = []
+    t0 = time.perf_counter()
+    a.apply(ops[:prologue], QSV_FLAG_FUSE)
+    b.apply(ops[:prologue], QSV_FLAG_FUSE)
+    for L in range(0, layers, 10):
+        chunk = ops[prologue + L * per_layer:prologue + min(layers, L + 10) * per_layer]
+        a.apply(chunk, QSV_FLAG_FUSE)
+        b.apply(chunk, QSV_FLAG_FUSE)
+        c = b.compare(a)
+        rows.append({"layers": min(layers, L + 10), "fidelity": c["fidelity"],
+                     "max_abs_diff": c["max_abs_diff"], "norm_drift_c64": abs(c["norm2_self"] - 1.0),
+                     "norm_drift_c128": abs(c["norm2_other"] - 1.0)})
+        log(f"bench: precision {rows[-1]}")
+    dt = time.perf_counter() - t0
+    a.destroy()
+    b.destroy()
+    return {"qubits": n, "layers": layers, "gates": int(len(ops)), "seconds": round(dt, 2), "per_10_layers": rows,
+            "min_fidelity": min(r["fidelity"] for r in rows)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +274,15 @@ def main():
     ap.add_argument("--no-fused", action="store_true", help="skip the fused tiled-pass measurement")
     ap.add_argument("--cpu-qubits", type=int, default=28)
     ap.add_argument("--sweep", metavar="PATH", help="also write GB/s per gate type x bit position to PATH")
+    ap.add_argument("--qaoa", type=int, default=4096, help="batched QAOA instances (0 to skip)")
+    ap.add_argument("--serve", type=int, default=120, help="served requests per arrival rate (0 to skip)")
+    ap.add_argument("--serve-plain", type=int, default=40, help="requests timed on the plain path")
+    ap.add_argument("--rates", default="50,200,1000,2000")
+    ap.add_argument("--precision-qubits", type=int, default=32, help="0 to skip the precision study")
+    ap.add_argument("--precision-layers", type=int, default=50)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also copy the state once with the library's copy kernel (a kernel of known bytes for "
+                         "scaling hardware byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -159,6 +330,11 @@ def main():
     dom = max(stats, key=lambda k: stats[k]["total_ms"])
     d = stats[dom]
     achieved = d["bytes"] / (d["total_ms"] / 1e3) / 1e9
+    if args.calibrate_copy:
+        other = engine.DeviceState(n, args.prec)
+        other.copy_from(dev)
+        other.destroy()
+        log(f"bench: calibration copy of {dev.nbytes} B done")
     fused = None
     if not args.no_fused:
         from qclojure_amd._abi import QSV_FLAG_FUSE
@@ -219,6 +395,21 @@ def main():
         "kernels": kern,
         "fused": fused,
     }
+    def section(name, fn):
+        try:
+            log(f"bench: {name} ...")
+            out[name] = fn()
+        except Exception as e:  # a section that cannot run on this box is reported, not fatal
+            out[name] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"bench: {name} failed: {e}")
+
+    if args.qaoa:
+        section("batched_qaoa", lambda: bench_batched_qaoa(engine, args.qaoa, args.seed, plain_every=16))
+    if args.serve:
+        rates = [float(r) for r in args.rates.split(",") if r]
+        section("serving", lambda: bench_serving(engine, rates, args.serve, args.serve_plain, args.seed))
+    if args.precision_qubits:
+        section("precision", lambda: bench_precision(engine, args.precision_qubits, args.precision_layers, args.seed))
     if args.sweep:
         table = position_sweep(engine, n, args.prec)
         with open(args.sweep, "w") as f:

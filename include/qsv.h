@@ -143,6 +143,10 @@ int qsv_expect_pauli(const qsv_state* s, const char* strings, uint64_t n_terms, 
 int qsv_expect_1q(const qsv_state* s, int qubit, const double m[8], double out[2]);
 /* inner product <a|b> with FP64 accumulation (state.clj:1164-1185 takes |.|). */
 int qsv_inner(const qsv_state* a, const qsv_state* b, double out[2]);
+/* One pass over two states of the same size and any precisions (precision
+ * studies): out = {Re <a|b>, Im <a|b>, sum |a_i|^2, sum |b_i|^2, max |a_i - b_i|}.
+ * state-fidelity (state.clj:1164-1185) is |<a|b>|; |<a|b>|^2 is the overlap fidelity. */
+int qsv_state_compare(const qsv_state* a, const qsv_state* b, double out[5]);
 
 /* Counter-based uniform in [0,1) used by qsv_sample / MEASURE: splitmix64 of
  * (seed + (k+1) * 0x9E3779B97F4A7C15), top 53 bits. Exposed so callers can

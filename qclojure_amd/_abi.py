@@ -125,6 +125,7 @@ SIGNATURES = {
     "qsv_expect_pauli": (_i32, [_vp, ctypes.c_char_p, _u64, _vp]),
     "qsv_expect_1q": (_i32, [_vp, _i32, _vp, _vp]),
     "qsv_inner": (_i32, [_vp, _vp, _vp]),
+    "qsv_state_compare": (_i32, [_vp, _vp, _vp]),
     "qsv_uniform": (_dbl, [_u64, _u64]),
     "qsv_batch_expect_z": (_i32, [ctypes.POINTER(QsvBatchDesc), _vp]),
     "qsv_server_create": (_i32, [ctypes.POINTER(QsvServerConfig), ctypes.POINTER(_vp)]),

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <condition_variable>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <mutex>
@@ -538,6 +539,20 @@ int launch_passes(qsv_state* s, std::vector<FPass>& passes)
   return QSV_OK;
 }
 
+// Index bits 0..L-1 are always tile bits, so a wave's loads are runs of 2^L
+// amplitudes. Defaults: 2^L * bytes = 1 KiB (a full wave of contiguous
+// amplitudes); QSV_TILE_LOWBITS overrides it (tuning only).
+int tile_low_bits(const qsv_state* s)
+{
+  static const int env = [] {
+    const char* v = std::getenv("QSV_TILE_LOWBITS");
+    return v ? std::atoi(v) : -1;
+  }();
+  const int TB = s->prec == QSV_C128 ? kTileBitsC128 : kTileBitsC64;
+  int L = env >= 0 ? env : (s->prec == QSV_C128 ? 6 : 7);
+  return std::max(low_bit_threshold(s), std::min(L, TB - 2));
+}
+
 // Build one pass from the chosen gates (program order) and the tile bit set.
 // Returns the number of chosen gates the pass holds (a suffix may be cut to
 // respect the stage limit).
@@ -545,8 +560,8 @@ size_t build_pass(const qsv_state* s, const std::vector<BGate>& gs, const std::v
                   uint64_t set, FPass& p)
 {
   const int TB = s->prec == QSV_C128 ? kTileBitsC128 : kTileBitsC64;
-  const int RB = TB - 8;
-  const int L = low_bit_threshold(s);
+  const int RB = 4;   // register bits per thread in the tile kernel
+  const int L = tile_low_bits(s);
   std::memset(&p, 0, sizeof(p));
   uint64_t tile = set;
   for (int b = 0; __builtin_popcountll(tile) < TB; ++b) tile |= 1ull << b;
@@ -575,9 +590,8 @@ size_t build_pass(const qsv_state* s, const std::vector<BGate>& gs, const std::v
         if ((g.bmask >> b) & 1ull) fg.bmask |= 1u << loc[b];
     fg.phase_only = g.phase_only ? 1 : 0;
     std::memcpy(fg.m, g.m, sizeof(fg.m));
-    uint32_t need = 0;
-    if (g.kind == FG_DENSE) need = 1u << fg.a;
-    if (g.kind == FG_SWAP) need = (1u << fg.a) | (1u << fg.b);
+    // only dense targets must be register bits (swaps go through LDS)
+    const uint32_t need = (g.kind == FG_DENSE) ? (1u << fg.a) : 0u;
     if (__builtin_popcount(cur | need) > RB) {
       if (nst == kMaxPassStages - 1) break;   // stage limit: cut the pass here
       p.stages[nst++] = FStage{cur, g0, (int32_t)i};
@@ -616,7 +630,7 @@ int run_fused(qsv_state* s, const std::vector<BGate>& gs)
     }
     return QSV_OK;
   }
-  const int L = low_bit_threshold(s);
+  const int L = tile_low_bits(s);
   const uint64_t lowmask = (1ull << L) - 1ull;
   const uint64_t all = (s->n >= 64) ? ~0ull : ((1ull << s->n) - 1ull);
   const size_t N = gs.size();
@@ -866,8 +880,24 @@ int qsv_state_copy(qsv_state* dst, const qsv_state* src)
   if (rc) return rc;
   if (dst->n != src->n || dst->prec != src->prec) return fail(QSV_E_INVALID, "copy: shape or precision mismatch");
   HIPCHK(hipStreamSynchronize(src->stream));
-  HIPCHK(hipMemcpyAsync(dst->amps, src->amps, src->S * src->bytes_per_amp(), hipMemcpyDeviceToDevice, dst->stream));
+  launch_copy(src->prec, src->amps, dst->amps, src->S, dst->stream);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(dst->stream));
+  return QSV_OK;
+}
+
+int qsv_state_compare(const qsv_state* a, const qsv_state* b, double out[5])
+{
+  int rc = check_state(a);
+  if (rc) return rc;
+  rc = check_state(b);
+  if (rc) return rc;
+  if (!out) return fail(QSV_E_INVALID, "null output");
+  if (a->n != b->n) return fail(QSV_E_INVALID, "compare: qubit counts differ");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  launch_compare(a->prec, a->amps, b->prec, b->amps, a->S, a->partials, a->dout, a->stream);
+  HIPCHK(hipMemcpyAsync(out, a->dout, sizeof(double) * 5, hipMemcpyDeviceToHost, a->stream));
+  HIPCHK(hipStreamSynchronize(a->stream));
   return QSV_OK;
 }
 

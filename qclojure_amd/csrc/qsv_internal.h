@@ -64,6 +64,11 @@ void launch_blockade(int prec, void* psi, uint64_t S, uint64_t bmask, const doub
 // qsv_kernels.hip
 void launch_fill_basis(int prec, void* psi, uint64_t S, uint64_t idx, hipStream_t st);
 void launch_scale(int prec, void* psi, uint64_t S, double f, hipStream_t st);
+void launch_copy(int prec, const void* src, void* dst, uint64_t S, hipStream_t st);
+// out[0..4] = Re <a|b>, Im <a|b>, sum |a|^2, sum |b|^2, max |a_i - b_i|;
+// partials holds 5 * kReduceBlocks doubles.
+void launch_compare(int pa, const void* a, int pb, const void* b, uint64_t S, double* partials, double* out,
+                    hipStream_t st);
 void launch_f64_to_c64(const double* src, void* dst, uint64_t count, hipStream_t st);
 void launch_c64_to_f64(const void* src, double* dst, uint64_t count, hipStream_t st);
 

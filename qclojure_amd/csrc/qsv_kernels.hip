@@ -24,6 +24,78 @@ __global__ __launch_bounds__(kBlock) void k_fill_basis(typename VT<R>::T* __rest
   }
 }
 
+// Plain streaming copy (non-temporal), 4 amplitudes per thread.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_copy(const typename VT<R>::T* __restrict__ src,
+                                                 typename VT<R>::T* __restrict__ dst, uint64_t S)
+{
+  typedef R V __attribute__((ext_vector_type(2)));
+  const V* s = reinterpret_cast<const V*>(src);
+  V* d = reinterpret_cast<V*>(dst);
+  const uint64_t base = (uint64_t)blockIdx.x * (kBlock * 4) + threadIdx.x;
+  V v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint64_t i = base + (uint64_t)k * kBlock;
+    if (i < S) v[k] = __builtin_nontemporal_load(s + i);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint64_t i = base + (uint64_t)k * kBlock;
+    if (i < S) __builtin_nontemporal_store(v[k], d + i);
+  }
+}
+
+// One pass over two states of any precisions: partials of Re/Im <a|b>,
+// sum |a|^2, sum |b|^2 (FP64) and max |a_i - b_i|.
+template <typename RA, typename RB>
+__global__ __launch_bounds__(kBlock) void k_compare(const typename VT<RA>::T* __restrict__ a,
+                                                    const typename VT<RB>::T* __restrict__ b, uint64_t S,
+                                                    double* __restrict__ partials)
+{
+  __shared__ double sh[kBlock / 64];
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  double re = 0.0, im = 0.0, na = 0.0, nb = 0.0, mx = 0.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < S; i += stride) {
+    const double ax = a[i].x, ay = a[i].y, bx = b[i].x, by = b[i].y;
+    re += ax * bx + ay * by;
+    im += ax * by - ay * bx;
+    na += ax * ax + ay * ay;
+    nb += bx * bx + by * by;
+    const double dx = ax - bx, dy = ay - by;
+    mx = fmax(mx, sqrt(dx * dx + dy * dy));
+  }
+  double vals[4] = {re, im, na, nb};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double v = vals[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) partials[(uint64_t)q * kReduceBlocks + blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    partials[4ull * kReduceBlocks + blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
+__global__ __launch_bounds__(kBlock) void k_final_max(const double* __restrict__ partials, double* __restrict__ out)
+{
+  __shared__ double sh[kBlock / 64];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < kReduceBlocks; i += kBlock) m = fmax(m, partials[i]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_scale(typename VT<R>::T* __restrict__ psi, uint64_t S, R f)
 {
@@ -408,6 +480,31 @@ void launch_fill_basis(int prec, void* psi, uint64_t S, uint64_t idx, hipStream_
     hipLaunchKernelGGL((k_fill_basis<double>), dim3(g), dim3(kBlock), 0, st, (double2*)psi, S, idx);
   else
     hipLaunchKernelGGL((k_fill_basis<float>), dim3(g), dim3(kBlock), 0, st, (float2*)psi, S, idx);
+}
+
+void launch_copy(int prec, const void* src, void* dst, uint64_t S, hipStream_t st)
+{
+  const unsigned g = (unsigned)((S + kBlock * 4 - 1) / (kBlock * 4));
+  if (prec == QSV_C128)
+    hipLaunchKernelGGL((k_copy<double>), dim3(g), dim3(kBlock), 0, st, (const double2*)src, (double2*)dst, S);
+  else
+    hipLaunchKernelGGL((k_copy<float>), dim3(g), dim3(kBlock), 0, st, (const float2*)src, (float2*)dst, S);
+}
+
+void launch_compare(int pa, const void* a, int pb, const void* b, uint64_t S, double* partials, double* out,
+                    hipStream_t st)
+{
+  const dim3 g(kReduceBlocks), blk(kBlock);
+  if (pa == QSV_C128 && pb == QSV_C128)
+    hipLaunchKernelGGL((k_compare<double, double>), g, blk, 0, st, (const double2*)a, (const double2*)b, S, partials);
+  else if (pa == QSV_C128)
+    hipLaunchKernelGGL((k_compare<double, float>), g, blk, 0, st, (const double2*)a, (const float2*)b, S, partials);
+  else if (pb == QSV_C128)
+    hipLaunchKernelGGL((k_compare<float, double>), g, blk, 0, st, (const float2*)a, (const double2*)b, S, partials);
+  else
+    hipLaunchKernelGGL((k_compare<float, float>), g, blk, 0, st, (const float2*)a, (const float2*)b, S, partials);
+  hipLaunchKernelGGL(k_final_sum, dim3(4), blk, 0, st, partials, out);
+  hipLaunchKernelGGL(k_final_max, dim3(1), blk, 0, st, partials + 4ull * kReduceBlocks, out + 4);
 }
 
 void launch_scale(int prec, void* psi, uint64_t S, double f, hipStream_t st)

@@ -173,6 +173,15 @@ class DeviceState:
         check(lib().qsv_inner(self.handle, other.handle, _ptr(out)))
         return complex(out[0], out[1])
 
+    def compare(self, other: "DeviceState") -> dict:
+        """One device pass over both states (any precisions): overlap fidelity
+        |<self|other>|^2, max |a_i - b_i| and both norms."""
+        out = np.empty(5, dtype=np.float64)
+        check(lib().qsv_state_compare(self.handle, other.handle, _ptr(out)))
+        ov = complex(out[0], out[1])
+        return {"overlap": ov, "fidelity": abs(ov) ** 2, "norm2_self": out[2], "norm2_other": out[3],
+                "max_abs_diff": out[4]}
+
     # -- timing ------------------------------------------------------------------
     def timing(self, enable: bool) -> None:
         check(lib().qsv_timing_enable(self.handle, 1 if enable else 0))

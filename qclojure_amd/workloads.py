@@ -44,8 +44,55 @@ def random_layer_circuit(n: int, layers: int, seed: int) -> dict:
 
 
 def random_layer_ops(n: int, layers: int, seed: int) -> np.ndarray:
-    """The same circuit decoded straight to qsv_op records."""
-    return qc.decode_circuit(random_layer_circuit(n, layers, seed))
+    """The same circuit as random_layer_circuit, written straight to qsv_op
+    records with the same random draws (no per-op Python objects)."""
+    rng = np.random.default_rng(seed)
+    per = n + n // 2
+    out = np.zeros(layers * per, dtype=OP_DTYPE)
+    kinds = np.array([OP["H"], OP["RX"], OP["RY"], OP["RZ"]], dtype=np.int32)
+    for L in range(layers):
+        choice = rng.integers(0, 4, size=n)
+        angles = rng.uniform(0.0, 2.0 * math.pi, size=n)
+        blk = out[L * per:L * per + n]
+        blk["kind"] = kinds[choice]
+        blk["q"][:, 0] = np.arange(n)
+        blk["angle"] = np.where(choice == 0, 0.0, angles)
+        perm = rng.permutation(n)
+        flips = rng.integers(0, 2, size=n // 2)
+        first, second = perm[0:2 * (n // 2):2], perm[1:2 * (n // 2):2]
+        sw = flips.astype(bool)
+        cn = out[L * per + n:(L + 1) * per]
+        cn["kind"] = OP["CNOT"]
+        cn["q"][:, 0] = np.where(sw, second, first)
+        cn["q"][:, 1] = np.where(sw, first, second)
+    return out
+
+
+def qaoa_ops(inst: dict) -> np.ndarray:
+    """A QAOA instance's ansatz (qaoa.clj:524-564) straight to qsv_op records:
+    H on every qubit; per (gamma, beta): for each edge term in Hamiltonian order
+    CNOT(q1,q2) RZ(q2, 2 gamma c) CNOT(q1,q2), then RX(q, 2 beta) on every qubit."""
+    n = inst["n"]
+    rows = [(OP["H"], q, 0.0, 0) for q in range(n)]
+    hp = inst["hamiltonian"]
+    ps = inst["params"]
+    for k in range(0, len(ps), 2):
+        gamma, beta = ps[k], ps[k + 1]
+        for t in hp:
+            idx = [i for i, ch in enumerate(t["pauli-string"]) if ch != "I"]
+            if not idx:
+                continue
+            q1, q2 = idx
+            ang = 2.0 * gamma * t["coefficient"]
+            rows += [(OP["CNOT"], q1, 0.0, q2), (OP["RZ"], q2, ang, 0), (OP["CNOT"], q1, 0.0, q2)]
+        for q in range(n):
+            rows.append((OP["RX"], q, 2.0 * beta * 1.0, 0))
+    out = np.zeros(len(rows), dtype=OP_DTYPE)
+    out["kind"] = [r[0] for r in rows]
+    out["q"][:, 0] = [r[1] for r in rows]
+    out["q"][:, 1] = [r[3] for r in rows]
+    out["angle"] = [r[2] for r in rows]
+    return out
 
 
 def layer_slices(n: int, layers: int) -> list[slice]:

@@ -74,8 +74,8 @@ def test_fused_random_programs_c64(eng, oracle, n):
     note(f"fused c64 n={n}")
     exp = oracle.apply_program(init, n, ops)
     got = _run(eng, n, ops, prec="c64", flags=QSV_FLAG_FUSE, init=init)
-    assert abs(np.vdot(exp, got)) ** 2 > 1 - 1e-9
-    assert np.max(np.abs(got - exp)) < 1e-6
+    assert abs(np.vdot(exp, got)) ** 2 > 1 - 1e-6   # the complex64 fidelity floor
+    assert np.max(np.abs(got - exp)) < 1e-5
 
 
 def test_fused_every_position(eng, oracle):

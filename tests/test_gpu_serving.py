@@ -38,7 +38,8 @@ def test_same_alone_and_in_the_mix(eng, prec):
         got = [srv.wait(i, r[2]) for i, r in zip(ids, reqs)]
         st = srv.stats()
     assert st["completed"] == len(reqs) and st["failed"] == 0
-    assert st["small_batched_requests"] >= 5
+    lds_n = 13 if prec == "c128" else 14   # largest state one workgroup's LDS holds
+    assert st["small_batched_requests"] == sum(1 for r in reqs if r[0] <= lds_n)
     for (n, ops, shots, seed), (info, outs) in zip(reqs, got):
         assert info["status"] == "completed" and info["latency-ms"] > 0
         ref, psi = _alone(eng, n, ops, shots, seed, prec)
