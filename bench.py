@@ -169,6 +169,7 @@ def bench_serving(engine, rates, n_req: int, n_plain: int, seed: int) -> dict:
     reqs = [(r["n"], wl.random_layer_ops(r["n"], r["depth"], r["seed"]), r["shots"], r["seed"]) for r in stream]
     unit_arrivals = [r["arrival-s"] for r in stream]
     gates = sum(len(r[1]) for r in reqs)
+    n_plain = n_plain if 0 < n_plain < n_req else n_req
     # plain: one request at a time, per-gate kernels, then shots
     service = []
     states = {}
@@ -252,6 +253,7 @@ def bench_precision(engine, n: int, layers: int, seed: int) -> dict:
         b.apply(chunk, QSV_FLAG_FUSE)
         c = b.compare(a)
         rows.append({"layers": min(layers, L + 10), "fidelity": c["fidelity"],
+                     "fidelity_normalized": c["fidelity"] / (c["norm2_self"] * c["norm2_other"]),
                      "max_abs_diff": c["max_abs_diff"], "norm_drift_c64": abs(c["norm2_self"] - 1.0),
                      "norm_drift_c128": abs(c["norm2_other"] - 1.0)})
         log(f"bench: precision {rows[-1]}")
@@ -259,7 +261,8 @@ def bench_precision(engine, n: int, layers: int, seed: int) -> dict:
     a.destroy()
     b.destroy()
     return {"qubits": n, "layers": layers, "gates": int(len(ops)), "seconds": round(dt, 2), "per_10_layers": rows,
-            "min_fidelity": min(r["fidelity"] for r in rows)}
+            "min_fidelity": min(r["fidelity"] for r in rows),
+            "min_fidelity_normalized": min(r["fidelity_normalized"] for r in rows)}
 
 
 def main():
@@ -276,8 +279,9 @@ def main():
     ap.add_argument("--sweep", metavar="PATH", help="also write GB/s per gate type x bit position to PATH")
     ap.add_argument("--qaoa", type=int, default=4096, help="batched QAOA instances (0 to skip)")
     ap.add_argument("--serve", type=int, default=120, help="served requests per arrival rate (0 to skip)")
-    ap.add_argument("--serve-plain", type=int, default=40, help="requests timed on the plain path")
-    ap.add_argument("--rates", default="50,200,1000,2000")
+    ap.add_argument("--serve-plain", type=int, default=0,
+                    help="requests timed on the plain path (0: the same requests as served)")
+    ap.add_argument("--rates", default="5,10,20,50,1000")
     ap.add_argument("--precision-qubits", type=int, default=32, help="0 to skip the precision study")
     ap.add_argument("--precision-layers", type=int, default=50)
     ap.add_argument("--calibrate-copy", action="store_true",

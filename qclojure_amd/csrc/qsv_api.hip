@@ -540,8 +540,9 @@ int launch_passes(qsv_state* s, std::vector<FPass>& passes)
 }
 
 // Index bits 0..L-1 are always tile bits, so a wave's loads are runs of 2^L
-// amplitudes. Defaults: 2^L * bytes = 1 KiB (a full wave of contiguous
-// amplitudes); QSV_TILE_LOWBITS overrides it (tuning only).
+// amplitudes. Longer runs stream better but leave fewer tile bits for gates;
+// L = 4 gave the shortest 30-qubit layers for both precisions.
+// QSV_TILE_LOWBITS overrides it (tuning only).
 int tile_low_bits(const qsv_state* s)
 {
   static const int env = [] {
@@ -549,7 +550,7 @@ int tile_low_bits(const qsv_state* s)
     return v ? std::atoi(v) : -1;
   }();
   const int TB = s->prec == QSV_C128 ? kTileBitsC128 : kTileBitsC64;
-  int L = env >= 0 ? env : (s->prec == QSV_C128 ? 6 : 7);
+  int L = env >= 0 ? env : 4;
   return std::max(low_bit_threshold(s), std::min(L, TB - 2));
 }
 

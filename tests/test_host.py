@@ -166,6 +166,23 @@ def test_hamiltonian_grouping():
     assert set(bases) == {"x", "z", "identity"}
 
 
+def test_fast_generators_match_circuit_maps():
+    """The benchmark's direct-to-record generators reproduce the circuit-map path."""
+    for (n, layers, seed) in [(12, 20, 1), (30, 3, 9), (7, 5, 4), (1, 2, 3)]:
+        assert np.array_equal(wl.random_layer_ops(n, layers, seed),
+                              qc.decode_circuit(wl.random_layer_circuit(n, layers, seed)))
+    for seed in range(4):
+        inst = wl.qaoa_instance(seed)
+        assert np.array_equal(wl.qaoa_ops(inst), qc.decode_circuit(inst["circuit"]))
+
+
+def test_precision_circuit_shape():
+    c = wl.precision_circuit(8, 3, seed=2)
+    kinds = [o["operation-type"] for o in c["operations"]]
+    assert kinds.count("crz") == 28 and kinds.count("swap") == 4
+    assert kinds.count("cz") == 3 * 4 and len(kinds) == 2 * 8 + (8 + 28 + 4) + 3 * (3 * 8 + 4)
+
+
 def test_workloads_deterministic():
     a = wl.random_layer_ops(30, 3, seed=9)
     b = wl.random_layer_ops(30, 3, seed=9)
