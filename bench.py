@@ -342,6 +342,10 @@ def main():
     fused = None
     if not args.no_fused:
         from qclojure_amd._abi import QSV_FLAG_FUSE
+        tuned = None
+        if "QSV_TILE_LOWBITS" not in os.environ:
+            tuned = engine.tune_fused(args.prec)
+            log(f"bench: tuned tile low bits {tuned}")
         for w in range(args.warmup):
             dev.apply(layer(w), QSV_FLAG_FUSE)
         dev.sync()
@@ -361,6 +365,8 @@ def main():
                  "tile_kernel_avg_ms": round(blk["total_ms"] / max(1, blk["launches"]), 4),
                  "tile_kernel_GB_s": round(blk["bytes"] / (blk["total_ms"] / 1e3) / 1e9, 1) if blk["launches"] else None,
                  "effective_GB_s": round(total_bytes / args.steps / (f_elapsed / args.steps) / 1e9, 1),
+                 "tile_low_bits": tuned["low_bits"] if tuned else int(os.environ["QSV_TILE_LOWBITS"]),
+                 "tuner_ms_per_26q_layer": {str(k): round(v, 3) for k, v in tuned["ms_per_layer"].items()} if tuned else None,
                  "note": "effective_GB_s = per-gate algorithmic bytes of the layer / fused layer time"}
         log(f"bench: fused {fused}")
     traffic = None

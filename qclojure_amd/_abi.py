@@ -68,6 +68,14 @@ class QsvKernelStat(ctypes.Structure):
                 ("bytes", ctypes.c_double)]
 
 
+QSV_TUNE_MAX = 8
+
+
+class QsvTuneResult(ctypes.Structure):
+    _fields_ = [("low_bits", ctypes.c_int32), ("n_candidates", ctypes.c_int32),
+                ("candidate", ctypes.c_int32 * QSV_TUNE_MAX), ("ms", ctypes.c_double * QSV_TUNE_MAX)]
+
+
 class QsvBatchDesc(ctypes.Structure):
     _fields_ = [("n_instances", ctypes.c_uint64),
                 ("num_qubits", ctypes.c_void_p),
@@ -136,6 +144,7 @@ SIGNATURES = {
     "qsv_server_destroy": (_i32, [_vp]),
     "qsv_timing_enable": (_i32, [_vp, _i32]),
     "qsv_timing_read": (_i32, [_vp, _vp, _i32]),
+    "qsv_tune_fused": (_i32, [_i32, _i32, _vp]),
 }
 
 

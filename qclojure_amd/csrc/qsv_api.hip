@@ -4,6 +4,7 @@
 // apply-gate-to-state does (src/org/soulspace/qclojure/domain/circuit.clj:933-1072)
 // and turns each into one streaming kernel on the state's own HIP stream.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -76,6 +77,7 @@ struct qsv_state {
   int ring_next = 0;
   PassSlot* ring_pending = nullptr;
   uint64_t passes_launched = 0;
+  int tile_low = -1;            // tile low-run width forced by the tuner (-1: none)
   uint64_t bytes_per_amp() const { return prec == QSV_C128 ? 16 : 8; }
 };
 
@@ -540,9 +542,12 @@ int launch_passes(qsv_state* s, std::vector<FPass>& passes)
 }
 
 // Index bits 0..L-1 are always tile bits, so a wave's loads are runs of 2^L
-// amplitudes. Longer runs stream better but leave fewer tile bits for gates;
-// L = 4 gave the shortest 30-qubit layers for both precisions.
-// QSV_TILE_LOWBITS overrides it (tuning only).
+// amplitudes. Longer runs stream better but leave fewer tile bits for gates.
+// Order of precedence: the tuner's per-state override while it measures,
+// QSV_TILE_LOWBITS, the tuned width of this precision, then 4 (the shortest
+// 30-qubit layers for both precisions in a sweep of 3..8).
+std::atomic<int> g_tuned_low[2] = {{-1}, {-1}};
+
 int tile_low_bits(const qsv_state* s)
 {
   static const int env = [] {
@@ -550,7 +555,8 @@ int tile_low_bits(const qsv_state* s)
     return v ? std::atoi(v) : -1;
   }();
   const int TB = s->prec == QSV_C128 ? kTileBitsC128 : kTileBitsC64;
-  int L = env >= 0 ? env : 4;
+  const int tuned = g_tuned_low[s->prec == QSV_C128 ? 0 : 1].load();
+  int L = s->tile_low >= 0 ? s->tile_low : env >= 0 ? env : tuned >= 0 ? tuned : 4;
   return std::max(low_bit_threshold(s), std::min(L, TB - 2));
 }
 
@@ -1815,6 +1821,69 @@ int qsv_timing_read(qsv_state* s, qsv_kernel_stat out[QSV_KCLASS_COUNT], int res
   if (rc) return rc;
   if (out) std::memcpy(out, s->stats, sizeof(s->stats));
   if (reset) std::memset(s->stats, 0, sizeof(s->stats));
+  return QSV_OK;
+}
+
+int qsv_tune_fused(int precision, int num_qubits, qsv_tune_result* out)
+{
+  if (!out) return fail(QSV_E_INVALID, "null output");
+  if (precision != QSV_C128 && precision != QSV_C64) return fail(QSV_E_INVALID, "unknown precision");
+  const int n = num_qubits ? num_qubits : 26;
+  if (n < 16 || n > 28) return fail(QSV_E_INVALID, "tuning state must have 16..28 qubits");
+  std::memset(out, 0, sizeof(*out));
+  qsv_state* s = nullptr;
+  int rc = qsv_state_create(n, precision, &s);
+  if (rc) return rc;
+  // the timed layer: the dense unitary RZ(0.7) RY(0.3) on every bit, then
+  // CNOTs on a fixed pseudo-random matching
+  std::vector<BGate> layer;
+  const double c = std::cos(0.15), sn = std::sin(0.15), zc = std::cos(0.35), zs = std::sin(0.35);
+  const double m[8] = {c * zc, -c * zs, -sn * zc, sn * zs, sn * zc, sn * zs, c * zc, c * zs};
+  std::vector<int> perm(n);
+  for (int b = 0; b < n; ++b) perm[b] = b;
+  uint64_t x = 0x2545F4914F6CDD1Dull;
+  for (int b = n - 1; b > 0; --b) {
+    x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+    std::swap(perm[b], perm[(int)(x % (uint64_t)(b + 1))]);
+  }
+  for (int b = 0; b < n; ++b) emit_ctl(layer, {}, b, m);
+  const double xm[8] = {0, 0, 1, 0, 1, 0, 0, 0};
+  for (int k = 0; k + 1 < n; k += 2) emit_ctl(layer, {perm[k]}, perm[k + 1], xm);
+  const int TB = precision == QSV_C128 ? kTileBitsC128 : kTileBitsC64;
+  const int lo = low_bit_threshold(s), hi = std::min(TB - 4, lo + QSV_TUNE_MAX - 1);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double best = 1e300;
+  int bestL = 4;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    rc = fail(QSV_E_DEVICE, "hipEventCreate failed");
+  } else {
+    for (int L = lo; L <= hi && !rc; ++L) {
+      s->tile_low = L;
+      double t = 1e300;
+      for (int rep = 0; rep < 3 && !rc; ++rep) {   // the first run is a warm-up
+        if (hipEventRecord(e0, s->stream) != hipSuccess) { rc = fail(QSV_E_DEVICE, "hipEventRecord failed"); break; }
+        if ((rc = run_fused(s, layer))) break;
+        float ms = 0.f;
+        if (hipEventRecord(e1, s->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+          rc = fail(QSV_E_DEVICE, "tuning timing failed");
+          break;
+        }
+        if (rep) t = std::min(t, (double)ms);
+      }
+      if (rc) break;
+      out->candidate[out->n_candidates] = L;
+      out->ms[out->n_candidates++] = t;
+      if (t < best) { best = t; bestL = L; }
+    }
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  const int drc = qsv_state_destroy(s);
+  if (rc) return rc;
+  if (drc) return drc;
+  out->low_bits = bestL;
+  g_tuned_low[precision == QSV_C128 ? 0 : 1].store(bestL);
   return QSV_OK;
 }
 

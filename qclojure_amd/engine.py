@@ -286,6 +286,16 @@ class Server:
             pass
 
 
+def tune_fused(precision: str = "c128", num_qubits: int = 26) -> dict:
+    """Time one fused random layer for every candidate tile low-run width and
+    make the fastest the default for `precision` (qsv_tune_fused)."""
+    init()
+    res = _abi.QsvTuneResult()
+    check(lib().qsv_tune_fused(PRECISIONS[precision], num_qubits, ctypes.byref(res)))
+    return {"low_bits": res.low_bits,
+            "ms_per_layer": {int(res.candidate[i]): float(res.ms[i]) for i in range(res.n_candidates)}}
+
+
 def uniform(seed: int, k: int) -> float:
     """Counter-based uniform shared with the device sampler (qsv_uniform)."""
     return lib().qsv_uniform(seed & (2**64 - 1), k)

@@ -118,3 +118,23 @@ def test_small_states_fall_back(eng, oracle):
     rng = np.random.default_rng(1)
     ops = _program(rng, n, 100)
     assert np.max(np.abs(_run(eng, n, ops, flags=QSV_FLAG_FUSE) - oracle.run(n, ops))) < 1e-12
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_tuner_picks_a_candidate_and_results_hold(eng, prec):
+    """qsv_tune_fused times every candidate width and keeps the fastest; the
+    fused path under the tuned width still equals the per-gate path."""
+    note(f"tune_fused prec={prec} n=20")
+    res = eng.tune_fused(prec, 20)
+    times = res["ms_per_layer"]
+    assert len(times) >= 2 and res["low_bits"] in times
+    assert all(t > 0 for t in times.values())
+    assert times[res["low_bits"]] == min(times.values())
+    n = 21
+    ops = wl.random_layer_ops(n, 6, seed=12)
+    a = _run(eng, n, ops, prec)
+    b = _run(eng, n, ops, prec, flags=QSV_FLAG_FUSE)
+    tol = 1e-12 if prec == "c128" else 1e-5
+    assert np.max(np.abs(a - b)) < tol
+    with pytest.raises(Exception, match="16..28"):
+        eng.tune_fused(prec, 40)
