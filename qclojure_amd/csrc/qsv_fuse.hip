@@ -13,11 +13,7 @@
 // phases act on any bits (per-slot predicates). Switching layouts between
 // stages is one LDS transpose; a swap-type gate is one LDS round trip whose
 // write addresses are permuted. Tile bits 0..L-1 are index bits 0..L-1, so the
-// lanes of a wave address whole 128 B runs of global memory. Workgroups are
-// persistent (as many as fit on the chip at once) and prefetch their next tile
-// into a second register set while the current tile's gates run.
-#include <algorithm>
-
+// lanes of a wave address whole 128 B runs of global memory.
 #include "qsv_internal.h"
 
 namespace qsv {
@@ -85,7 +81,7 @@ __device__ __forceinline__ void reg_dense(T (&x)[kRegs], const M8& m, uint32_t k
 // 64 KiB of LDS per workgroup leaves room for two workgroups per CU.
 template <typename R, int TB>
 __global__ __launch_bounds__(1 << (TB - kRegBits)) void k_tile(typename FV<R>::T* __restrict__ psi,
-                                                               const FPass* __restrict__ pass, uint64_t ntiles)
+                                                               const FPass* __restrict__ pass)
 {
   using T = typename FV<R>::T;
   constexpr uint32_t FULL = (1u << TB) - 1u;
@@ -93,23 +89,19 @@ __global__ __launch_bounds__(1 << (TB - kRegBits)) void k_tile(typename FV<R>::T
 
   const uint32_t t = threadIdx.x;
   // tile base index: zeros inserted at the tile's global bits (ascending)
-  auto tile_base = [&](uint64_t tile) -> uint64_t {
-    uint64_t b = tile;
-    for (int i = 0; i < TB; ++i) {
-      const int p = pass->sorted_pos[i];
-      const uint64_t lo = b & ((1ull << p) - 1ull);
-      b = ((b ^ lo) << 1) | lo;
-    }
-    return b;
-  };
+  uint64_t base = blockIdx.x;
+  for (int i = 0; i < TB; ++i) {
+    const int p = pass->sorted_pos[i];
+    const uint64_t lo = base & ((1ull << p) - 1ull);
+    base = ((base ^ lo) << 1) | lo;
+  }
   uint64_t gbit[TB];  // global index bit of each tile-local bit
 #pragma unroll
   for (int i = 0; i < TB; ++i) gbit[i] = 1ull << pass->pos[i];
 
   // layout: tl = thread part of the local index, r0..r3 = local bit of
   // register bits 0..3
-  const uint32_t load_mask = pass->load_mask, store_mask = pass->store_mask;
-  uint32_t reg = load_mask;
+  uint32_t reg = pass->load_mask;
   uint32_t tl, r0, r1, r2, r3;
   auto set_layout = [&](uint32_t mask) {
     tl = pdep32(t, FULL & ~mask);
@@ -129,148 +121,116 @@ __global__ __launch_bounds__(1 << (TB - kRegBits)) void k_tile(typename FV<R>::T
       if (l & (1u << i)) g |= gbit[i];
     return g;
   };
-  // global addresses of the 16 slots under the current layout
-  auto gaddr = [&](uint64_t base, uint64_t (&ga)[kRegs]) {
+  // global addresses of the 16 slots: thread part | register-bit parts
+  auto gaddr = [&](uint64_t (&ga)[kRegs]) {
     const uint64_t gt = base | goff(tl);
     const uint64_t g0 = goff(r0), g1 = goff(r1), g2 = goff(r2), g3 = goff(r3);
 #pragma unroll
     for (int k = 0; k < kRegs; ++k)
       ga[k] = gt | ((k & 1) ? g0 : 0ull) | ((k & 2) ? g1 : 0ull) | ((k & 4) ? g2 : 0ull) | ((k & 8) ? g3 : 0ull);
   };
-  auto transpose_to = [&](T (&x)[kRegs], uint32_t mask) {
-#pragma unroll
-    for (int k = 0; k < kRegs; ++k) lds[swz(kl(k))] = x[k];
-    __syncthreads();
-    reg = mask;
-    set_layout(reg);
-#pragma unroll
-    for (int k = 0; k < kRegs; ++k) x[k] = lds[swz(kl(k))];
-    __syncthreads();
-  };
 
-  // Persistent loop over tiles; the next tile's loads are issued before the
-  // current tile's gates run, so HBM traffic overlaps the arithmetic.
-  uint64_t tile = blockIdx.x;
-  if (tile >= ntiles) return;
-  T x[kRegs], nx[kRegs];
-  set_layout(load_mask);
+  T x[kRegs];
+  set_layout(reg);
   {
     uint64_t ga[kRegs];
-    gaddr(tile_base(tile), ga);
+    gaddr(ga);
 #pragma unroll
     for (int k = 0; k < kRegs; ++k) x[k] = __builtin_nontemporal_load(psi + ga[k]);
   }
+
   const int nst = pass->nstages;
-  while (true) {
-    const uint64_t next = tile + gridDim.x;
-    const bool has_next = next < ntiles;
-    // reg == load_mask here
-    if (has_next) {
-      uint64_t ga[kRegs];
-      gaddr(tile_base(next), ga);
+  for (int s = 0; s < nst; ++s) {
+    const FStage st = pass->stages[s];
+    if (st.regmask != reg) {
 #pragma unroll
-      for (int k = 0; k < kRegs; ++k) nx[k] = __builtin_nontemporal_load(psi + ga[k]);
+      for (int k = 0; k < kRegs; ++k) lds[swz(kl(k))] = x[k];
+      __syncthreads();
+      reg = st.regmask;
+      set_layout(reg);
+#pragma unroll
+      for (int k = 0; k < kRegs; ++k) x[k] = lds[swz(kl(k))];
+      __syncthreads();
     }
-    for (int s = 0; s < nst; ++s) {
-      const FStage st = pass->stages[s];
-      if (st.regmask != reg) transpose_to(x, st.regmask);
-      for (int gi = st.g0; gi < st.g1; ++gi) {
-        const FGate* g = &pass->gates[gi];
-        const int kind = g->kind;
-        const uint32_t cm = g->cmask;
-        if (kind == FG_DENSE) {
-          // register-resident controls become a slot mask, the others a
-          // per-thread predicate
-          const uint32_t cthr = cm & ~reg;
-          if ((tl & cthr) != cthr) continue;
-          const uint32_t kc = ((cm & r0) ? 1u : 0u) | ((cm & r1) ? 2u : 0u) | ((cm & r2) ? 4u : 0u) |
-                              ((cm & r3) ? 8u : 0u);
-          const M8 m{g->m[0], g->m[1], g->m[2], g->m[3], g->m[4], g->m[5], g->m[6], g->m[7]};
-          const uint32_t tb = 1u << g->a;
-          if (tb == r0) reg_dense<0, T, R>(x, m, kc);
-          else if (tb == r1) reg_dense<1, T, R>(x, m, kc);
-          else if (tb == r2) reg_dense<2, T, R>(x, m, kc);
-          else reg_dense<3, T, R>(x, m, kc);
-        } else if (kind == FG_PHASE) {
-          const uint32_t tb = 1u << g->a;
-          const double d0r = g->m[0], d0i = g->m[1], d1r = g->m[6], d1i = g->m[7];
-          const bool skip0 = g->phase_only != 0;
+    for (int gi = st.g0; gi < st.g1; ++gi) {
+      const FGate* g = &pass->gates[gi];
+      const int kind = g->kind;
+      const uint32_t cm = g->cmask;
+      if (kind == FG_DENSE) {
+        // register-resident controls become a slot mask, the others a
+        // per-thread predicate
+        const uint32_t cthr = cm & ~reg;
+        if ((tl & cthr) != cthr) continue;
+        const uint32_t kc = ((cm & r0) ? 1u : 0u) | ((cm & r1) ? 2u : 0u) | ((cm & r2) ? 4u : 0u) |
+                            ((cm & r3) ? 8u : 0u);
+        const M8 m{g->m[0], g->m[1], g->m[2], g->m[3], g->m[4], g->m[5], g->m[6], g->m[7]};
+        const uint32_t tb = 1u << g->a;
+        if (tb == r0) reg_dense<0, T, R>(x, m, kc);
+        else if (tb == r1) reg_dense<1, T, R>(x, m, kc);
+        else if (tb == r2) reg_dense<2, T, R>(x, m, kc);
+        else reg_dense<3, T, R>(x, m, kc);
+      } else if (kind == FG_PHASE) {
+        const uint32_t tb = 1u << g->a;
+        const double d0r = g->m[0], d0i = g->m[1], d1r = g->m[6], d1i = g->m[7];
+        const bool skip0 = g->phase_only != 0;
 #pragma unroll
-          for (int k = 0; k < kRegs; ++k) {
-            const uint32_t l = kl(k);
-            if ((l & cm) == cm) {
-              if (l & tb) x[k] = cscale<T, R>(d1r, d1i, x[k]);
-              else if (!skip0) x[k] = cscale<T, R>(d0r, d0i, x[k]);
-            }
+        for (int k = 0; k < kRegs; ++k) {
+          const uint32_t l = kl(k);
+          if ((l & cm) == cm) {
+            if (l & tb) x[k] = cscale<T, R>(d1r, d1i, x[k]);
+            else if (!skip0) x[k] = cscale<T, R>(d0r, d0i, x[k]);
           }
-        } else if (kind == FG_SWAP) {
-          // exchange local bits a and b (times ph) where the controls are
-          // set: every thread writes its slots to permuted LDS addresses
-          const uint32_t ba = 1u << g->a, bb = 1u << g->b;
-          const double pr = g->m[0], pi = g->m[1];
-#pragma unroll
-          for (int k = 0; k < kRegs; ++k) {
-            const uint32_t l = kl(k);
-            const bool mv = ((l & cm) == cm) && (((l & ba) != 0) != ((l & bb) != 0));
-            lds[swz(mv ? (l ^ ba ^ bb) : l)] = mv ? cscale<T, R>(pr, pi, x[k]) : x[k];
-          }
-          __syncthreads();
-#pragma unroll
-          for (int k = 0; k < kRegs; ++k) x[k] = lds[swz(kl(k))];
-          __syncthreads();
-        } else {  // FG_BLOCKADE: phase where exactly one bit of bmask is set
-          const uint32_t bm = g->bmask;
-          const double pr = g->m[0], pi = g->m[1];
-#pragma unroll
-          for (int k = 0; k < kRegs; ++k)
-            if (__popc(kl(k) & bm) == 1) x[k] = cscale<T, R>(pr, pi, x[k]);
         }
+      } else if (kind == FG_SWAP) {
+        // exchange local bits a and b (times ph) where the controls are set:
+        // every thread writes its slots to permuted LDS addresses
+        const uint32_t ba = 1u << g->a, bb = 1u << g->b;
+        const double pr = g->m[0], pi = g->m[1];
+#pragma unroll
+        for (int k = 0; k < kRegs; ++k) {
+          const uint32_t l = kl(k);
+          const bool mv = ((l & cm) == cm) && (((l & ba) != 0) != ((l & bb) != 0));
+          lds[swz(mv ? (l ^ ba ^ bb) : l)] = mv ? cscale<T, R>(pr, pi, x[k]) : x[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kRegs; ++k) x[k] = lds[swz(kl(k))];
+        __syncthreads();
+      } else {  // FG_BLOCKADE: phase where exactly one bit of bmask is set
+        const uint32_t bm = g->bmask;
+        const double pr = g->m[0], pi = g->m[1];
+#pragma unroll
+        for (int k = 0; k < kRegs; ++k)
+          if (__popc(kl(k) & bm) == 1) x[k] = cscale<T, R>(pr, pi, x[k]);
       }
     }
-    if (reg != store_mask) transpose_to(x, store_mask);
-    {
-      uint64_t ga[kRegs];
-      gaddr(tile_base(tile), ga);
+  }
+
+  if (reg != pass->store_mask) {
 #pragma unroll
-      for (int k = 0; k < kRegs; ++k) __builtin_nontemporal_store(x[k], psi + ga[k]);
-    }
-    if (!has_next) break;
-    tile = next;
-    if (reg != load_mask) {
-      // the prefetched tile was loaded in the load layout
-      reg = load_mask;
-      set_layout(reg);
-    }
+    for (int k = 0; k < kRegs; ++k) lds[swz(kl(k))] = x[k];
+    __syncthreads();
+    reg = pass->store_mask;
+    set_layout(reg);
 #pragma unroll
-    for (int k = 0; k < kRegs; ++k) x[k] = nx[k];
+    for (int k = 0; k < kRegs; ++k) x[k] = lds[swz(kl(k))];
+  }
+  {
+    uint64_t ga[kRegs];
+    gaddr(ga);
+#pragma unroll
+    for (int k = 0; k < kRegs; ++k) __builtin_nontemporal_store(x[k], psi + ga[k]);
   }
 }
 
 void launch_tile(int prec, void* psi, const FPass* dpass, uint64_t ntiles, hipStream_t st)
 {
-  // as many persistent workgroups as can be resident at once, each looping
-  // over tiles
-  static int resident[2] = {0, 0};
-  int& res = resident[prec == QSV_C128 ? 0 : 1];
-  if (!res) {
-    int dev = 0, cus = 0, per_cu = 0;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (prec == QSV_C128)
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_tile<double, kTileBitsC128>),
-                                                   1 << (kTileBitsC128 - kRegBits), 0);
-    else
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_tile<float, kTileBitsC64>),
-                                                   1 << (kTileBitsC64 - kRegBits), 0);
-    res = std::max(1, cus) * std::max(1, per_cu);
-  }
-  const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)res);
   if (prec == QSV_C128)
-    hipLaunchKernelGGL((k_tile<double, kTileBitsC128>), dim3((unsigned)grid), dim3(1 << (kTileBitsC128 - kRegBits)),
-                       0, st, (FV<double>::T*)psi, dpass, ntiles);
+    hipLaunchKernelGGL((k_tile<double, kTileBitsC128>), dim3((unsigned)ntiles), dim3(1 << (kTileBitsC128 - kRegBits)),
+                       0, st, (FV<double>::T*)psi, dpass);
   else
-    hipLaunchKernelGGL((k_tile<float, kTileBitsC64>), dim3((unsigned)grid), dim3(1 << (kTileBitsC64 - kRegBits)),
-                       0, st, (FV<float>::T*)psi, dpass, ntiles);
+    hipLaunchKernelGGL((k_tile<float, kTileBitsC64>), dim3((unsigned)ntiles), dim3(1 << (kTileBitsC64 - kRegBits)),
+                       0, st, (FV<float>::T*)psi, dpass);
 }
 
 }  // namespace qsv

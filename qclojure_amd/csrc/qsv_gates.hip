@@ -279,13 +279,27 @@ void launch_gate1(int prec, void* psi, const GateArgs& g, const double m[8], hip
   Mat2<double> M;
   for (int k = 0; k < 8; ++k) M.m[k] = m[k];
   if (g.lo_t == 0) {
-    constexpr int PPT = 2;
-    const unsigned grid = blocks_for(g.ins.count, PPT);
-    if (prec == QSV_C128)
-      hipLaunchKernelGGL((k_gate1<double, PPT, false>), dim3(grid), dim3(kBlock), 0, st,
-                         (NV<double>::T*)psi, g, M);
-    else
-      hipLaunchKernelGGL((k_gate1<float, PPT, false>), dim3(grid), dim3(kBlock), 0, st, (NV<float>::T*)psi, g, M);
+    // Pair partners 64 KiB or more apart stream faster with four pairs in
+    // flight per thread; closer partners with two (measured on 30-qubit
+    // states: 5.7-5.8 vs 5.5-5.6 TB/s above, 5.8 vs 5.45 TB/s below).
+    const uint64_t dist = g.tbit * (prec == QSV_C128 ? 16u : 8u);
+    if (dist >= (64u << 10)) {
+      constexpr int PPT = 4;
+      const unsigned grid = blocks_for(g.ins.count, PPT);
+      if (prec == QSV_C128)
+        hipLaunchKernelGGL((k_gate1<double, PPT, false>), dim3(grid), dim3(kBlock), 0, st,
+                           (NV<double>::T*)psi, g, M);
+      else
+        hipLaunchKernelGGL((k_gate1<float, PPT, false>), dim3(grid), dim3(kBlock), 0, st, (NV<float>::T*)psi, g, M);
+    } else {
+      constexpr int PPT = 2;
+      const unsigned grid = blocks_for(g.ins.count, PPT);
+      if (prec == QSV_C128)
+        hipLaunchKernelGGL((k_gate1<double, PPT, false>), dim3(grid), dim3(kBlock), 0, st,
+                           (NV<double>::T*)psi, g, M);
+      else
+        hipLaunchKernelGGL((k_gate1<float, PPT, false>), dim3(grid), dim3(kBlock), 0, st, (NV<float>::T*)psi, g, M);
+    }
   } else {
     constexpr int PPT = 4;
     const unsigned grid = blocks_for(g.ins.count, PPT);
