@@ -78,6 +78,9 @@ typedef struct qsv_op {
 #define QSV_FLAG_MSB_SWAP  0x1u  /* SWAP/ISWAP on MSB bit positions. Default is the
                                     reference's LSB positions (gate.clj:768-778,823-833). */
 #define QSV_FLAG_FUSE      0x2u  /* batch runs of gates into LDS-tiled passes */
+#define QSV_FLAG_FP64_PASSES 0x4u /* complex64 tiled passes compute in FP64. By default
+                                    they compute in FP32 and remove the resulting norm
+                                    drift pass by pass (FP64 per-tile norm sums). */
 
 /* ---- library / device --------------------------------------------------- */
 typedef struct qsv_device_info {

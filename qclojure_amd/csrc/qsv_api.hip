@@ -78,6 +78,10 @@ struct qsv_state {
   PassSlot* ring_pending = nullptr;
   uint64_t passes_launched = 0;
   int tile_low = -1;            // tile low-run width forced by the tuner (-1: none)
+  double* norm_buf = nullptr;   // complex64 FP32 passes: per-tile norms (2 x ntiles) + control (2)
+  uint64_t* samp_dev = nullptr;   // serving: reusable shot buffers (grow only)
+  uint64_t* samp_host = nullptr;  // pinned
+  uint64_t samp_cap = 0;
   uint64_t bytes_per_amp() const { return prec == QSV_C128 ? 16 : 8; }
 };
 
@@ -521,19 +525,30 @@ int upload_passes(qsv_state* s, const std::vector<FPass>& passes, FPass** dev)
   return QSV_OK;
 }
 
-int launch_passes(qsv_state* s, std::vector<FPass>& passes)
+int launch_passes(qsv_state* s, std::vector<FPass>& passes, bool f32)
 {
   if (passes.empty()) return QSV_OK;
+  const int TB = s->prec == QSV_C128 ? kTileBitsC128 : kTileBitsC64;
+  const uint64_t ntiles = s->S >> TB;
+  TileNorm nb{nullptr, nullptr, nullptr};
+  if (f32) {
+    if (!s->norm_buf) HIPCHK(hipMalloc((void**)&s->norm_buf, (2 * ntiles + 2) * sizeof(double)));
+    nb = TileNorm{s->norm_buf, s->norm_buf + ntiles, s->norm_buf + 2 * ntiles};
+  }
   FPass* dev = nullptr;
   int rc = upload_passes(s, passes, &dev);
   if (rc) return rc;
-  const int TB = s->prec == QSV_C128 ? kTileBitsC128 : kTileBitsC64;
-  const uint64_t ntiles = s->S >> TB;
   for (size_t i = 0; i < passes.size(); ++i) {
     const FPass* p = dev + i;
     rc = timed(s, QSV_KCLASS_BLOCK, 2.0 * s->S * s->bytes_per_amp(),
-               [&] { launch_tile(s->prec, s->amps, p, ntiles, s->stream); });
+               [&] { launch_tile(s->prec, f32, s->amps, p, ntiles, nb, s->stream); });
     if (rc) return rc;
+    if (passes[i].renorm & FP_RENORM_OUT) {
+      const bool first = (passes[i].renorm & FP_RENORM_IN) != 0;
+      rc = timed(s, QSV_KCLASS_OTHER, (first ? 2.0 : 1.0) * ntiles * sizeof(double),
+                 [&] { launch_norm_ctl(nb, ntiles, first, s->stream); });
+      if (rc) return rc;
+    }
   }
   HIPCHK(hipEventRecord(s->ring_pending->event, s->stream));
   s->passes_launched += passes.size();
@@ -627,9 +642,20 @@ size_t build_pass(const qsv_state* s, const std::vector<BGate>& gs, const std::v
   return taken;
 }
 
-int run_fused(qsv_state* s, const std::vector<BGate>& gs)
+// The last pass of a run of passes needs no scale for a successor.
+void end_run(std::vector<FPass>& passes)
+{
+  if (passes.empty()) return;
+  int& rn = passes.back().renorm;
+  rn &= ~FP_RENORM_OUT;
+  if (!(rn & FP_RENORM_APPLY)) rn = 0;
+}
+
+// flags: QSV_FLAG_FP64_PASSES keeps complex64 pass arithmetic in FP64.
+int run_fused(qsv_state* s, const std::vector<BGate>& gs, uint32_t flags)
 {
   const int TB = s->prec == QSV_C128 ? kTileBitsC128 : kTileBitsC64;
+  const bool f32 = s->prec == QSV_C64 && !(flags & QSV_FLAG_FP64_PASSES);
   if (s->n < TB) {
     for (const auto& g : gs) {
       int rc = run_direct(s, g);
@@ -645,15 +671,18 @@ int run_fused(qsv_state* s, const std::vector<BGate>& gs)
   std::vector<FPass> passes;
   std::vector<size_t> chosen;
   size_t first = 0;
+  bool run_first = true;   // the next pass starts a run (renormalisation target)
   int rc;
   while (true) {
     while (first < N && done[first]) ++first;
     if (first >= N) break;
     const BGate& g0 = gs[first];
     if (g0.kind == BG_MEASURE || __builtin_popcountll(gate_bits(g0) | lowmask) > TB) {
-      if ((rc = launch_passes(s, passes))) return rc;
+      end_run(passes);
+      if ((rc = launch_passes(s, passes, f32))) return rc;
       if ((rc = run_direct(s, g0))) return rc;
       done[first] = 1;
+      run_first = true;
       continue;
     }
     uint64_t set = lowmask, blocked = 0;
@@ -679,10 +708,13 @@ int run_fused(qsv_state* s, const std::vector<BGate>& gs)
     FPass p;
     const size_t taken = build_pass(s, gs, chosen, set, p);
     for (size_t i = 0; i < taken; ++i) done[chosen[i]] = 1;
+    p.renorm = f32 ? (FP_RENORM_OUT | (run_first ? FP_RENORM_IN : FP_RENORM_APPLY)) : 0;
+    run_first = false;
     passes.push_back(p);
-    if (passes.size() >= 512 && (rc = launch_passes(s, passes))) return rc;
+    if (passes.size() >= 512 && (rc = launch_passes(s, passes, f32))) return rc;
   }
-  return launch_passes(s, passes);
+  end_run(passes);
+  return launch_passes(s, passes, f32);
 }
 
 int check_state(const qsv_state* s)
@@ -807,6 +839,9 @@ int qsv_state_destroy(qsv_state* s)
   if (s->dout) hipFree(s->dout);
   if (s->prefix) hipFree(s->prefix);
   if (s->hist) hipFree(s->hist);
+  if (s->norm_buf) hipFree(s->norm_buf);
+  if (s->samp_dev) hipFree(s->samp_dev);
+  if (s->samp_host) hipHostFree(s->samp_host);
   if (s->stream) hipStreamDestroy(s->stream);
   delete s;
   return QSV_OK;
@@ -933,7 +968,7 @@ int qsv_apply_program(qsv_state* s, const qsv_op* ops, uint64_t n_ops, uint32_t 
     }
   }
   if (flags & QSV_FLAG_FUSE) {
-    rc = run_fused(s, gates);
+    rc = run_fused(s, gates, flags);
   } else {
     rc = QSV_OK;
     for (const auto& g : gates)
@@ -1291,7 +1326,7 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
       return fail(QSV_E_DEVICE, "batch: stream wait failed");
     }
     launch_fill_basis(s->prec, s->amps, s->S, 0, s->stream);
-    if ((rc = run_fused(s, big_gates[k]))) {
+    if ((rc = run_fused(s, big_gates[k], 0))) {
       cleanup();
       return rc;
     }
@@ -1391,7 +1426,11 @@ struct qsv_server {
   uint64_t next_id = 1;
   bool stop = false;
   std::thread th;
+  // Finished requests' states are kept for reuse (a hipFree waits for the
+  // whole device, which would stall every request in flight) up to
+  // budget / 5 bytes; beyond that they are freed.
   std::vector<qsv_state*> free_states[40];
+  uint64_t cached_bytes = 0;
   static constexpr int kSlots = 4;
   SmallSlot slots[kSlots];
   qsv_server_stats stats{};
@@ -1408,12 +1447,16 @@ struct qsv_server {
 
 namespace {
 
+uint64_t state_bytes(const qsv_state* s) { return s->S * s->bytes_per_amp(); }
+
+// Only the scheduler thread touches free_states and cached_bytes.
 int take_state(qsv_server* sv, int n, qsv_state** out)
 {
   auto& fl = sv->free_states[n];
   if (!fl.empty()) {
     *out = fl.back();
     fl.pop_back();
+    sv->cached_bytes -= state_bytes(*out);
     return QSV_OK;
   }
   return qsv_state_create(n, sv->cfg.precision, out);
@@ -1421,9 +1464,13 @@ int take_state(qsv_server* sv, int n, qsv_state** out)
 
 void give_state(qsv_server* sv, qsv_state* s)
 {
-  auto& fl = sv->free_states[s->n];
-  if (fl.size() < 2) fl.push_back(s);
-  else qsv_state_destroy(s);
+  const uint64_t b = state_bytes(s);
+  if (sv->cached_bytes + b <= sv->budget / 5) {
+    sv->free_states[s->n].push_back(s);
+    sv->cached_bytes += b;
+  } else {
+    qsv_state_destroy(s);
+  }
 }
 
 void finish(qsv_server* sv, SrvReq* r, int status, const std::string& err)
@@ -1523,17 +1570,27 @@ int launch_large(qsv_server* sv, SrvReq* r)
   if (rc) return rc;
   qsv_state* s = r->st;
   launch_fill_basis(s->prec, s->amps, s->S, 0, s->stream);
-  if (sv->cfg.flags & QSV_FLAG_FUSE) rc = run_fused(s, r->gates);
+  if (sv->cfg.flags & QSV_FLAG_FUSE) rc = run_fused(s, r->gates, sv->cfg.flags);
   else {
     for (const auto& g : r->gates)
       if ((rc = run_direct(s, g))) break;
   }
   if (rc) return rc;
   if (!s->prefix) HIPCHK(hipMalloc(&s->prefix, std::max<uint64_t>(1, s->S >> kSampleChunkLog) * sizeof(double)));
-  if (r->shots) {
-    HIPCHK(hipMalloc(&r->d_out, r->shots * sizeof(uint64_t)));
-    HIPCHK(hipHostMalloc((void**)&r->h_out, r->shots * sizeof(uint64_t), hipHostMallocDefault));
+  if (r->shots > s->samp_cap) {
+    // grow the state's shot buffers (the state is idle: it was just taken)
+    if (s->samp_dev) HIPCHK(hipFree(s->samp_dev));
+    if (s->samp_host) HIPCHK(hipHostFree(s->samp_host));
+    s->samp_dev = nullptr;
+    s->samp_host = nullptr;
+    s->samp_cap = 0;
+    const uint64_t cap = std::max<uint64_t>(r->shots, 1024);
+    HIPCHK(hipMalloc((void**)&s->samp_dev, cap * sizeof(uint64_t)));
+    HIPCHK(hipHostMalloc((void**)&s->samp_host, cap * sizeof(uint64_t), hipHostMallocDefault));
+    s->samp_cap = cap;
   }
+  r->d_out = s->samp_dev;
+  r->h_out = s->samp_host;
   launch_sample(s->prec, s->amps, s->S, s->prefix, r->shots, r->seed, r->d_out, s->stream);
   if (r->shots)
     HIPCHK(hipMemcpyAsync(r->h_out, r->d_out, r->shots * sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream));
@@ -1545,9 +1602,8 @@ int launch_large(qsv_server* sv, SrvReq* r)
 
 void release_large(qsv_server* sv, SrvReq* r)
 {
+  // the shot buffers belong to the state, which goes back to the cache
   if (r->ev) hipEventDestroy(r->ev);
-  if (r->d_out) hipFree(r->d_out);
-  if (r->h_out) hipHostFree(r->h_out);
   r->ev = nullptr;
   r->d_out = nullptr;
   r->h_out = nullptr;
@@ -1862,7 +1918,7 @@ int qsv_tune_fused(int precision, int num_qubits, qsv_tune_result* out)
       double t = 1e300;
       for (int rep = 0; rep < 3 && !rc; ++rep) {   // the first run is a warm-up
         if (hipEventRecord(e0, s->stream) != hipSuccess) { rc = fail(QSV_E_DEVICE, "hipEventRecord failed"); break; }
-        if ((rc = run_fused(s, layer))) break;
+        if ((rc = run_fused(s, layer, 0))) break;
         float ms = 0.f;
         if (hipEventRecord(e1, s->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
             hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {

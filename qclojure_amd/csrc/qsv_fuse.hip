@@ -14,6 +14,10 @@
 // stages is one LDS transpose; a swap-type gate is one LDS round trip whose
 // write addresses are permuted. Tile bits 0..L-1 are index bits 0..L-1, so the
 // lanes of a wave address whole 128 B runs of global memory.
+//
+// Arithmetic type A: FP64 for complex128; FP32 or FP64 for complex64. With
+// FP32 the pass also reports per-tile norms and applies the previous pass's
+// renormalisation scale at load (TileNorm in qsv_internal.h).
 #include "qsv_internal.h"
 
 namespace qsv {
@@ -39,14 +43,15 @@ __device__ __forceinline__ uint32_t pdep32(uint32_t src, uint32_t mask)
 // rewrites the low 4 bits.
 __device__ __forceinline__ uint32_t swz(uint32_t l) { return l ^ (((l >> 4) ^ (l >> 8)) & 15u); }
 
+template <typename A>
 struct M8 {
-  double m0, m1, m2, m3, m4, m5, m6, m7;
+  A m0, m1, m2, m3, m4, m5, m6, m7;
 };
 
-template <typename T, typename R>
-__device__ __forceinline__ void cmad_pair(const M8& m, T& a, T& b)
+template <typename T, typename R, typename A>
+__device__ __forceinline__ void cmad_pair(const M8<A>& m, T& a, T& b)
 {
-  const double ax = a.x, ay = a.y, bx = b.x, by = b.y;
+  const A ax = a.x, ay = a.y, bx = b.x, by = b.y;
   T y0, y1;
   y0.x = (R)(m.m0 * ax - m.m1 * ay + m.m2 * bx - m.m3 * by);
   y0.y = (R)(m.m0 * ay + m.m1 * ax + m.m2 * by + m.m3 * bx);
@@ -56,10 +61,10 @@ __device__ __forceinline__ void cmad_pair(const M8& m, T& a, T& b)
   b = y1;
 }
 
-template <typename T, typename R>
-__device__ __forceinline__ T cscale(double re, double im, T a)
+template <typename T, typename R, typename A>
+__device__ __forceinline__ T cscale(A re, A im, T a)
 {
-  const double ax = a.x, ay = a.y;
+  const A ax = a.x, ay = a.y;
   T y;
   y.x = (R)(re * ax - im * ay);
   y.y = (R)(re * ay + im * ax);
@@ -67,25 +72,51 @@ __device__ __forceinline__ T cscale(double re, double im, T a)
 }
 
 // Dense 2x2 on register bit RBIT; slot k acts when (k & kc) == kc.
-template <int RBIT, typename T, typename R>
-__device__ __forceinline__ void reg_dense(T (&x)[kRegs], const M8& m, uint32_t kc)
+template <int RBIT, typename T, typename R, typename A>
+__device__ __forceinline__ void reg_dense(T (&x)[kRegs], const M8<A>& m, uint32_t kc)
 {
 #pragma unroll
   for (int k = 0; k < kRegs; ++k) {
     if (!(k & (1 << RBIT))) {
-      if ((k & kc) == kc) cmad_pair<T, R>(m, x[k], x[k | (1 << RBIT)]);
+      if ((k & kc) == kc) cmad_pair<T, R, A>(m, x[k], x[k | (1 << RBIT)]);
     }
   }
 }
 
+// Sum of v over the workgroup in a fixed order (valid in thread 0).
+__device__ __forceinline__ double block_sum(double v, double* red)
+{
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const uint32_t t = threadIdx.x;
+  if ((t & 63u) == 0) red[t >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (t == 0)
+    for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) s += red[w];
+  return s;
+}
+
+template <typename T>
+__device__ __forceinline__ double norm2_slots(const T (&x)[kRegs])
+{
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < kRegs; ++k) {
+    const double re = x[k].x, im = x[k].y;
+    s += re * re + im * im;
+  }
+  return s;
+}
+
 // 64 KiB of LDS per workgroup leaves room for two workgroups per CU.
-template <typename R, int TB>
+template <typename R, typename A, int TB>
 __global__ __launch_bounds__(1 << (TB - kRegBits)) void k_tile(typename FV<R>::T* __restrict__ psi,
-                                                               const FPass* __restrict__ pass)
+                                                               const FPass* __restrict__ pass, TileNorm nb)
 {
   using T = typename FV<R>::T;
   constexpr uint32_t FULL = (1u << TB) - 1u;
   __shared__ T lds[1 << TB];
+  __shared__ double red[(1 << (TB - kRegBits)) / 64];
 
   const uint32_t t = threadIdx.x;
   // tile base index: zeros inserted at the tile's global bits (ascending)
@@ -138,6 +169,20 @@ __global__ __launch_bounds__(1 << (TB - kRegBits)) void k_tile(typename FV<R>::T
 #pragma unroll
     for (int k = 0; k < kRegs; ++k) x[k] = __builtin_nontemporal_load(psi + ga[k]);
   }
+  const int rn = pass->renorm;
+  if (rn & FP_RENORM_APPLY) {
+    const A sc = (A)nb.ctl[1];
+#pragma unroll
+    for (int k = 0; k < kRegs; ++k) {
+      x[k].x = (R)(sc * (A)x[k].x);
+      x[k].y = (R)(sc * (A)x[k].y);
+    }
+  }
+  if (rn & FP_RENORM_IN) {
+    const double v = block_sum(norm2_slots(x), red);
+    if (t == 0) nb.part_in[blockIdx.x] = v;
+    __syncthreads();
+  }
 
   const int nst = pass->nstages;
   for (int s = 0; s < nst; ++s) {
@@ -163,34 +208,35 @@ __global__ __launch_bounds__(1 << (TB - kRegBits)) void k_tile(typename FV<R>::T
         if ((tl & cthr) != cthr) continue;
         const uint32_t kc = ((cm & r0) ? 1u : 0u) | ((cm & r1) ? 2u : 0u) | ((cm & r2) ? 4u : 0u) |
                             ((cm & r3) ? 8u : 0u);
-        const M8 m{g->m[0], g->m[1], g->m[2], g->m[3], g->m[4], g->m[5], g->m[6], g->m[7]};
+        const M8<A> m{(A)g->m[0], (A)g->m[1], (A)g->m[2], (A)g->m[3],
+                      (A)g->m[4], (A)g->m[5], (A)g->m[6], (A)g->m[7]};
         const uint32_t tb = 1u << g->a;
-        if (tb == r0) reg_dense<0, T, R>(x, m, kc);
-        else if (tb == r1) reg_dense<1, T, R>(x, m, kc);
-        else if (tb == r2) reg_dense<2, T, R>(x, m, kc);
-        else reg_dense<3, T, R>(x, m, kc);
+        if (tb == r0) reg_dense<0, T, R, A>(x, m, kc);
+        else if (tb == r1) reg_dense<1, T, R, A>(x, m, kc);
+        else if (tb == r2) reg_dense<2, T, R, A>(x, m, kc);
+        else reg_dense<3, T, R, A>(x, m, kc);
       } else if (kind == FG_PHASE) {
         const uint32_t tb = 1u << g->a;
-        const double d0r = g->m[0], d0i = g->m[1], d1r = g->m[6], d1i = g->m[7];
+        const A d0r = (A)g->m[0], d0i = (A)g->m[1], d1r = (A)g->m[6], d1i = (A)g->m[7];
         const bool skip0 = g->phase_only != 0;
 #pragma unroll
         for (int k = 0; k < kRegs; ++k) {
           const uint32_t l = kl(k);
           if ((l & cm) == cm) {
-            if (l & tb) x[k] = cscale<T, R>(d1r, d1i, x[k]);
-            else if (!skip0) x[k] = cscale<T, R>(d0r, d0i, x[k]);
+            if (l & tb) x[k] = cscale<T, R, A>(d1r, d1i, x[k]);
+            else if (!skip0) x[k] = cscale<T, R, A>(d0r, d0i, x[k]);
           }
         }
       } else if (kind == FG_SWAP) {
         // exchange local bits a and b (times ph) where the controls are set:
         // every thread writes its slots to permuted LDS addresses
         const uint32_t ba = 1u << g->a, bb = 1u << g->b;
-        const double pr = g->m[0], pi = g->m[1];
+        const A pr = (A)g->m[0], pi = (A)g->m[1];
 #pragma unroll
         for (int k = 0; k < kRegs; ++k) {
           const uint32_t l = kl(k);
           const bool mv = ((l & cm) == cm) && (((l & ba) != 0) != ((l & bb) != 0));
-          lds[swz(mv ? (l ^ ba ^ bb) : l)] = mv ? cscale<T, R>(pr, pi, x[k]) : x[k];
+          lds[swz(mv ? (l ^ ba ^ bb) : l)] = mv ? cscale<T, R, A>(pr, pi, x[k]) : x[k];
         }
         __syncthreads();
 #pragma unroll
@@ -198,10 +244,10 @@ __global__ __launch_bounds__(1 << (TB - kRegBits)) void k_tile(typename FV<R>::T
         __syncthreads();
       } else {  // FG_BLOCKADE: phase where exactly one bit of bmask is set
         const uint32_t bm = g->bmask;
-        const double pr = g->m[0], pi = g->m[1];
+        const A pr = (A)g->m[0], pi = (A)g->m[1];
 #pragma unroll
         for (int k = 0; k < kRegs; ++k)
-          if (__popc(kl(k) & bm) == 1) x[k] = cscale<T, R>(pr, pi, x[k]);
+          if (__popc(kl(k) & bm) == 1) x[k] = cscale<T, R, A>(pr, pi, x[k]);
       }
     }
   }
@@ -221,16 +267,59 @@ __global__ __launch_bounds__(1 << (TB - kRegBits)) void k_tile(typename FV<R>::T
 #pragma unroll
     for (int k = 0; k < kRegs; ++k) __builtin_nontemporal_store(x[k], psi + ga[k]);
   }
+  if (rn & FP_RENORM_OUT) {
+    const double v = block_sum(norm2_slots(x), red);
+    if (t == 0) nb.part_out[blockIdx.x] = v;
+  }
 }
 
-void launch_tile(int prec, void* psi, const FPass* dpass, uint64_t ntiles, hipStream_t st)
+// One workgroup: fixed-order sums of the per-tile norms, then the scale the
+// next pass applies at load.
+__global__ __launch_bounds__(1024) void k_norm_ctl(const double* __restrict__ part_in,
+                                                   const double* __restrict__ part_out, uint64_t ntiles, int first,
+                                                   double* __restrict__ ctl)
+{
+  __shared__ double sa[1024], sb[1024];
+  const uint32_t t = threadIdx.x;
+  double a = 0.0, b = 0.0;
+  for (uint64_t i = t; i < ntiles; i += 1024) {
+    if (first) a += part_in[i];
+    b += part_out[i];
+  }
+  sa[t] = a;
+  sb[t] = b;
+  __syncthreads();
+  for (uint32_t w = 512; w > 0; w >>= 1) {
+    if (t < w) {
+      sa[t] += sa[t + w];
+      sb[t] += sb[t + w];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    const double target = first ? sa[0] : ctl[0];
+    ctl[0] = target;
+    ctl[1] = (target > 0.0 && sb[0] > 0.0) ? sqrt(target / sb[0]) : 1.0;
+  }
+}
+
+void launch_tile(int prec, bool f32, void* psi, const FPass* dpass, uint64_t ntiles, const TileNorm& nb,
+                 hipStream_t st)
 {
   if (prec == QSV_C128)
-    hipLaunchKernelGGL((k_tile<double, kTileBitsC128>), dim3((unsigned)ntiles), dim3(1 << (kTileBitsC128 - kRegBits)),
-                       0, st, (FV<double>::T*)psi, dpass);
+    hipLaunchKernelGGL((k_tile<double, double, kTileBitsC128>), dim3((unsigned)ntiles),
+                       dim3(1 << (kTileBitsC128 - kRegBits)), 0, st, (FV<double>::T*)psi, dpass, nb);
+  else if (f32)
+    hipLaunchKernelGGL((k_tile<float, float, kTileBitsC64>), dim3((unsigned)ntiles),
+                       dim3(1 << (kTileBitsC64 - kRegBits)), 0, st, (FV<float>::T*)psi, dpass, nb);
   else
-    hipLaunchKernelGGL((k_tile<float, kTileBitsC64>), dim3((unsigned)ntiles), dim3(1 << (kTileBitsC64 - kRegBits)),
-                       0, st, (FV<float>::T*)psi, dpass);
+    hipLaunchKernelGGL((k_tile<float, double, kTileBitsC64>), dim3((unsigned)ntiles),
+                       dim3(1 << (kTileBitsC64 - kRegBits)), 0, st, (FV<float>::T*)psi, dpass, nb);
+}
+
+void launch_norm_ctl(const TileNorm& nb, uint64_t ntiles, bool first, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_norm_ctl, dim3(1), dim3(1024), 0, st, nb.part_in, nb.part_out, ntiles, first ? 1 : 0, nb.ctl);
 }
 
 }  // namespace qsv

@@ -117,17 +117,36 @@ struct FStage {
   int32_t g0, g1;      // gates [g0, g1)
 };
 
+// Complex64 passes compute in FP32 by default. The rounded 2x2 matrices are
+// unitary only to ~1e-7, so the norm would drift by that much per gate; it is
+// removed pass by pass instead: a pass sums |amplitude|^2 per tile in FP64
+// (FP_RENORM_OUT, and FP_RENORM_IN for the input of the first pass of a run),
+// a one-workgroup kernel turns the sums into scale = sqrt(target / norm^2),
+// and the next pass multiplies its loads by it (FP_RENORM_APPLY).
+enum { FP_RENORM_APPLY = 1, FP_RENORM_IN = 2, FP_RENORM_OUT = 4 };
+
+struct TileNorm {
+  double* part_in;    // per-tile input norm^2 (first pass of a run)
+  double* part_out;   // per-tile output norm^2
+  double* ctl;        // [0] target norm^2, [1] scale for the next pass
+};
+
 struct FPass {
   int32_t nstages;
   int32_t ngates;
   uint32_t load_mask, store_mask;   // register masks used for the global load / store
+  int32_t renorm;                   // FP_RENORM_* bits
+  int32_t pad;
   int32_t pos[16];                  // global index bit of tile-local bit i
   int32_t sorted_pos[16];           // the same bits in ascending order
   FStage stages[kMaxPassStages];
   FGate gates[kMaxPassGates];
 };
 
-void launch_tile(int prec, void* psi, const FPass* dpass, uint64_t ntiles, hipStream_t st);
+// f32: complex64 arithmetic in FP32 (ignored for complex128, always FP64)
+void launch_tile(int prec, bool f32, void* psi, const FPass* dpass, uint64_t ntiles, const TileNorm& nb,
+                 hipStream_t st);
+void launch_norm_ctl(const TileNorm& nb, uint64_t ntiles, bool first, hipStream_t st);
 
 // ---- batched small circuits (qsv_batch.hip) -----------------------------------
 constexpr int kBatchThreads = 256;

@@ -138,3 +138,22 @@ def test_tuner_picks_a_candidate_and_results_hold(eng, prec):
     assert np.max(np.abs(a - b)) < tol
     with pytest.raises(Exception, match="16..28"):
         eng.tune_fused(prec, 40)
+
+
+def test_c64_fp32_passes_renormalise(eng):
+    """complex64 passes computed in FP32 keep the norm (the per-pass
+    renormalisation removes the drift of the rounded matrices) and stay within
+    the stated fidelity of the FP64 passes and of complex128."""
+    from qclojure_amd._abi import QSV_FLAG_FP64_PASSES
+    n = 22
+    ops = wl.random_layer_ops(n, 60, seed=31)
+    note(f"c64 fp32 passes n={n} gates={len(ops)}")
+    with eng.DeviceState(n, "c128") as ref, eng.DeviceState(n, "c64") as f32, \
+            eng.DeviceState(n, "c64") as f64:
+        ref.apply(ops, QSV_FLAG_FUSE)
+        f32.apply(ops, QSV_FLAG_FUSE)
+        f64.apply(ops, QSV_FLAG_FUSE | QSV_FLAG_FP64_PASSES)
+        assert abs(f32.norm2() - 1.0) < 2e-6
+        for s in (f32, f64):
+            c = s.compare(ref)
+            assert c["fidelity"] > 1 - 1e-6 and c["max_abs_diff"] < 1e-5, c
