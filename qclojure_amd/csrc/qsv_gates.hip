@@ -55,6 +55,18 @@ __device__ __forceinline__ T shfl2(T v, int src)
   return r;
 }
 
+// Workgroup -> chunk of the index space. The dispatcher hands consecutive
+// workgroups to the 8 XCDs in turn; with XCD = true each XCD streams one
+// contiguous eighth of the chunks instead (a bijection when gridDim.x is a
+// multiple of 8, the identity otherwise).
+template <bool XCD>
+__device__ __forceinline__ uint64_t block_chunk()
+{
+  const uint32_t g = gridDim.x, b = blockIdx.x;
+  if (!XCD || (g & 7u)) return b;
+  return (uint64_t)(b & 7u) * (g >> 3) + (b >> 3);
+}
+
 // (m_r + i m_i) * (x + i y) + (n_r + i n_i) * (u + i v), FP64, rounded to R
 template <typename T, typename R>
 __device__ __forceinline__ T cmad2(double mr, double mi, T a, double nr, double ni, T b)
@@ -80,11 +92,11 @@ __device__ __forceinline__ T cmul1(double dr, double di, T a)
 // Dense 2x2 (optionally controlled). TLOW=false: target bit is high (inserted),
 // each thread slot holds the pair. TLOW=true: target bit is a lane bit.
 // ---------------------------------------------------------------------------
-template <typename R, int PPT, bool TLOW>
+template <typename R, int PPT, bool TLOW, bool XCD>
 __global__ __launch_bounds__(kBlock) void k_gate1(typename NV<R>::T* __restrict__ psi, GateArgs g, Mat2<double> M)
 {
   using T = typename NV<R>::T;
-  const uint64_t base = (uint64_t)blockIdx.x * (kBlock * PPT) + threadIdx.x;
+  const uint64_t base = block_chunk<XCD>() * (kBlock * PPT) + threadIdx.x;
   const uint64_t count = g.ins.count;
   uint64_t id[PPT];
   if constexpr (!TLOW) {
@@ -148,7 +160,7 @@ __global__ __launch_bounds__(kBlock) void k_phase(typename NV<R>::T* __restrict_
                                                    Cplx<double> d1)
 {
   using T = typename NV<R>::T;
-  const uint64_t base = (uint64_t)blockIdx.x * (kBlock * EPT) + threadIdx.x;
+  const uint64_t base = block_chunk<true>() * (kBlock * EPT) + threadIdx.x;
   const uint64_t count = g.ins.count;
   T x[EPT];
   uint64_t id[EPT];
@@ -248,7 +260,7 @@ __global__ __launch_bounds__(kBlock) void k_blockade(typename NV<R>::T* __restri
                                                       Cplx<double> ph)
 {
   using T = typename NV<R>::T;
-  const uint64_t base = (uint64_t)blockIdx.x * (kBlock * EPT) + threadIdx.x;
+  const uint64_t base = block_chunk<true>() * (kBlock * EPT) + threadIdx.x;
   T x[EPT];
 #pragma unroll
   for (int k = 0; k < EPT; ++k) {
@@ -279,34 +291,37 @@ void launch_gate1(int prec, void* psi, const GateArgs& g, const double m[8], hip
   Mat2<double> M;
   for (int k = 0; k < 8; ++k) M.m[k] = m[k];
   if (g.lo_t == 0) {
-    // Pair partners 64 KiB or more apart stream faster with four pairs in
-    // flight per thread; closer partners with two (measured on 30-qubit
-    // states: 5.7-5.8 vs 5.5-5.6 TB/s above, 5.8 vs 5.45 TB/s below).
+    // Pair partners 4 MiB or more apart stream fastest with four pairs in
+    // flight per thread in plain workgroup order; closer partners with two
+    // pairs and XCD-contiguous chunks (30-qubit probe, TB/s: >= 4 MiB 5.66-5.73
+    // vs 5.44-5.77; below, 5.83-5.97 vs 5.48-5.83).
     const uint64_t dist = g.tbit * (prec == QSV_C128 ? 16u : 8u);
-    if (dist >= (64u << 10)) {
+    if (dist >= (4u << 20)) {
       constexpr int PPT = 4;
       const unsigned grid = blocks_for(g.ins.count, PPT);
       if (prec == QSV_C128)
-        hipLaunchKernelGGL((k_gate1<double, PPT, false>), dim3(grid), dim3(kBlock), 0, st,
+        hipLaunchKernelGGL((k_gate1<double, PPT, false, false>), dim3(grid), dim3(kBlock), 0, st,
                            (NV<double>::T*)psi, g, M);
       else
-        hipLaunchKernelGGL((k_gate1<float, PPT, false>), dim3(grid), dim3(kBlock), 0, st, (NV<float>::T*)psi, g, M);
+        hipLaunchKernelGGL((k_gate1<float, PPT, false, false>), dim3(grid), dim3(kBlock), 0, st,
+                           (NV<float>::T*)psi, g, M);
     } else {
       constexpr int PPT = 2;
       const unsigned grid = blocks_for(g.ins.count, PPT);
       if (prec == QSV_C128)
-        hipLaunchKernelGGL((k_gate1<double, PPT, false>), dim3(grid), dim3(kBlock), 0, st,
+        hipLaunchKernelGGL((k_gate1<double, PPT, false, true>), dim3(grid), dim3(kBlock), 0, st,
                            (NV<double>::T*)psi, g, M);
       else
-        hipLaunchKernelGGL((k_gate1<float, PPT, false>), dim3(grid), dim3(kBlock), 0, st, (NV<float>::T*)psi, g, M);
+        hipLaunchKernelGGL((k_gate1<float, PPT, false, true>), dim3(grid), dim3(kBlock), 0, st,
+                           (NV<float>::T*)psi, g, M);
     }
   } else {
     constexpr int PPT = 4;
     const unsigned grid = blocks_for(g.ins.count, PPT);
     if (prec == QSV_C128)
-      hipLaunchKernelGGL((k_gate1<double, PPT, true>), dim3(grid), dim3(kBlock), 0, st, (NV<double>::T*)psi, g, M);
+      hipLaunchKernelGGL((k_gate1<double, PPT, true, true>), dim3(grid), dim3(kBlock), 0, st, (NV<double>::T*)psi, g, M);
     else
-      hipLaunchKernelGGL((k_gate1<float, PPT, true>), dim3(grid), dim3(kBlock), 0, st, (NV<float>::T*)psi, g, M);
+      hipLaunchKernelGGL((k_gate1<float, PPT, true, true>), dim3(grid), dim3(kBlock), 0, st, (NV<float>::T*)psi, g, M);
   }
 }
 
