@@ -175,6 +175,33 @@ typedef struct qsv_batch_desc {
 
 int qsv_batch_expect_z(const qsv_batch_desc* desc, double* energies);
 
+/* ---- noisy trajectories (the hardware simulator's shots) -------------------
+ * execute-single-noisy-shot-with-trajectory repeated n_traj times
+ * (src/org/soulspace/qclojure/adapter/backend/hardware_simulator.clj:80-104),
+ * each trajectory on its own |0..0> state: every op, then the noise channel of
+ * each site whose after_op is that op (noise.clj:53-103). A site's operator j
+ * is chosen as apply-quantum-channel does (channel.clj:185-222): weights
+ * w_j = max |k_j,rc|^2, the first j whose running sum exceeds the uniform u,
+ * else the last one (a single operator always). It is applied to `qubit` and
+ * the state renormalised (channel.clj:135-183). The measurement is one
+ * measure-state draw (state.clj:869-913) -> outcomes[t] (basis index).
+ * Uniforms: site k of trajectory t uses qsv_uniform(seed, t*QSV_TRAJ_SLOTS + k),
+ * its measurement qsv_uniform(seed, t*QSV_TRAJ_SLOTS + QSV_TRAJ_SLOTS - 1).
+ * final_states (optional) receives the first n_keep trajectories' normalised
+ * final states, 2^n interleaved FP64 (re, im) each. Sites must be ordered by
+ * after_op; mid-circuit MEASURE ops are rejected. */
+#define QSV_MAX_KRAUS   4
+#define QSV_TRAJ_SLOTS  (1ull << 20)
+typedef struct qsv_kraus_site {
+  uint64_t after_op;               /* index into ops                          */
+  int32_t  qubit;                  /* reference qubit number                  */
+  int32_t  n_kraus;                /* 1..QSV_MAX_KRAUS                        */
+  double   m[QSV_MAX_KRAUS][8];    /* row-major 2x2 complex (re, im) each     */
+} qsv_kraus_site;
+int qsv_noisy_trajectories(int num_qubits, int precision, const qsv_op* ops, uint64_t n_ops,
+                           const qsv_kraus_site* sites, uint64_t n_sites, uint64_t n_traj, uint64_t seed,
+                           uint64_t* outcomes, uint64_t n_keep, double* final_states);
+
 /* ---- serving: continuous batching of circuit requests ---------------------
  * The asynchronous job model of QuantumBackend (submit-circuit / job-status /
  * job-result, application/backend.clj:72-112; the reference runs each job in

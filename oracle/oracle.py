@@ -134,3 +134,46 @@ def expect_1q(psi: np.ndarray, n: int, q: int, m: np.ndarray) -> complex:
 
 def uniform(seed: int, k: int) -> float:
     return lib().ref_uniform(seed, k)
+
+
+TRAJ_SLOTS = 1 << 20
+_PHI = 0x9E3779B97F4A7C15
+
+
+def trajectories(n: int, ops: np.ndarray, sites, n_traj: int, seed: int, make_u):
+    """Noisy shots as the hardware simulator takes them
+    (src/org/soulspace/qclojure/adapter/backend/hardware_simulator.clj:80-104):
+    per trajectory, every op, then each site whose after-op is that op: the
+    operator chosen by apply-quantum-channel's rule (channel.clj:185-222:
+    weights max |k_ij|^2, first running sum above u, else the last) applied
+    UNSCALED and the state renormalised (apply-single-qubit-kraus-operator,
+    channel.clj:135-183); finally one measure-state draw. Site k of
+    trajectory t draws uniform(seed, t*2^20 + k), the measurement
+    uniform(seed, t*2^20 + 2^20 - 1). make_u(qubit, m) builds the explicit
+    2x2 op record. Returns (outcomes, normalised final states)."""
+    outs = np.zeros(n_traj, dtype=np.uint64)
+    finals = np.zeros((n_traj, 1 << n), dtype=np.complex128)
+    for t in range(n_traj):
+        psi = zero_state(n)
+        k = 0
+        for i in range(len(ops)):
+            psi = apply_program(psi, n, ops[i:i + 1])
+            while k < len(sites) and sites[k]["after-op"] == i:
+                kr = sites[k]["kraus"]
+                j = 0
+                if len(kr) > 1:
+                    u = uniform(seed, t * TRAJ_SLOTS + k)
+                    acc = 0.0
+                    for j in range(len(kr)):
+                        acc += float(np.max(np.abs(np.asarray(kr[j])) ** 2))
+                        if u < acc or j == len(kr) - 1:
+                            break
+                psi = apply_program(psi, n, make_u(sites[k]["qubit"], np.asarray(kr[j], dtype=np.complex128)))
+                nrm = np.sqrt(norm2(psi, n))
+                if nrm > 1e-10:
+                    psi = psi / nrm
+                k += 1
+        seed_m = (seed + (t * TRAJ_SLOTS + TRAJ_SLOTS - 1) * _PHI) % (1 << 64)
+        outs[t] = sample(psi, n, 1, seed_m)[0]
+        finals[t] = psi / np.sqrt(norm2(psi, n))
+    return outs, finals

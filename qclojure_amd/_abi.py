@@ -77,6 +77,15 @@ class QsvTuneResult(ctypes.Structure):
                 ("candidate", ctypes.c_int32 * QSV_TUNE_MAX), ("ms", ctypes.c_double * QSV_TUNE_MAX)]
 
 
+QSV_MAX_KRAUS = 4
+QSV_TRAJ_SLOTS = 1 << 20
+
+
+class QsvKrausSite(ctypes.Structure):
+    _fields_ = [("after_op", ctypes.c_uint64), ("qubit", ctypes.c_int32), ("n_kraus", ctypes.c_int32),
+                ("m", (ctypes.c_double * 8) * QSV_MAX_KRAUS)]
+
+
 class QsvBatchDesc(ctypes.Structure):
     _fields_ = [("n_instances", ctypes.c_uint64),
                 ("num_qubits", ctypes.c_void_p),
@@ -146,6 +155,7 @@ SIGNATURES = {
     "qsv_timing_enable": (_i32, [_vp, _i32]),
     "qsv_timing_read": (_i32, [_vp, _vp, _i32]),
     "qsv_tune_fused": (_i32, [_i32, _i32, _vp]),
+    "qsv_noisy_trajectories": (_i32, [_i32, _i32, _vp, _u64, _vp, _u64, _u64, _u64, _vp, _u64, _vp]),
 }
 
 

@@ -4,6 +4,7 @@
 // apply-gate-to-state does (src/org/soulspace/qclojure/domain/circuit.clj:933-1072)
 // and turns each into one streaming kernel on the state's own HIP stream.
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -1343,6 +1344,228 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
   hipEventDestroy(uploaded);
   if (e != hipSuccess) return fail(QSV_E_DEVICE, std::string("batch: ") + hipGetErrorString(e));
   return QSV_OK;
+}
+
+int qsv_noisy_trajectories(int num_qubits, int precision, const qsv_op* ops, uint64_t n_ops,
+                           const qsv_kraus_site* sites, uint64_t n_sites, uint64_t n_traj, uint64_t seed,
+                           uint64_t* outcomes, uint64_t n_keep, double* final_states)
+{
+  const int n = num_qubits;
+  if (n < 1 || n > 34) return fail(QSV_E_INVALID, "num_qubits out of range");
+  if (precision != QSV_C128 && precision != QSV_C64) return fail(QSV_E_INVALID, "unknown precision");
+  if ((n_ops && !ops) || (n_sites && !sites) || (n_traj && !outcomes) || (n_keep && !final_states))
+    return fail(QSV_E_INVALID, "null argument");
+  if (n_keep > n_traj) return fail(QSV_E_INVALID, "n_keep exceeds n_traj");
+  if (n_sites >= QSV_TRAJ_SLOTS) return fail(QSV_E_INVALID, "too many noise sites");
+  if (n_traj == 0) return QSV_OK;
+  int rc = ensure_init();
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(g_device));
+  // clean gates of each op, in order
+  std::vector<BGate> clean;
+  std::vector<size_t> op_end(n_ops);
+  uint64_t mk = 0;
+  for (uint64_t i = 0; i < n_ops; ++i) {
+    if (ops[i].kind == QSV_OP_MEASURE)
+      return fail(QSV_E_UNSUPPORTED, "op " + std::to_string(i) + ": mid-circuit measurements are not supported in trajectories");
+    rc = decode_op(n, ops[i], 0, 0, &mk, clean);
+    if (rc) {
+      g_err = "op " + std::to_string(i) + ": " + g_err;
+      return rc;
+    }
+    op_end[i] = clean.size();
+  }
+  // each site's operators scaled to max |k_ij| = 1 (the reference renormalises
+  // after applying one, so any positive multiple gives the same state) and
+  // their selection weights max |k_ij|^2 (channel.clj:203-212)
+  std::vector<std::vector<std::array<double, 8>>> kops(n_sites);
+  std::vector<std::vector<double>> kw(n_sites);
+  for (uint64_t k = 0; k < n_sites; ++k) {
+    const qsv_kraus_site& st = sites[k];
+    if (st.after_op >= n_ops) return fail(QSV_E_INVALID, "site " + std::to_string(k) + ": after_op out of range");
+    if (k && st.after_op < sites[k - 1].after_op) return fail(QSV_E_INVALID, "sites must be ordered by after_op");
+    if (st.qubit < 0 || st.qubit >= n) return fail(QSV_E_INVALID, "site " + std::to_string(k) + ": qubit out of range");
+    if (st.n_kraus < 1 || st.n_kraus > QSV_MAX_KRAUS)
+      return fail(QSV_E_INVALID, "site " + std::to_string(k) + ": n_kraus must be 1.." + std::to_string(QSV_MAX_KRAUS));
+    for (int j = 0; j < st.n_kraus; ++j) {
+      double mx = 0.0;
+      for (int e = 0; e < 4; ++e) mx = std::max(mx, st.m[j][2 * e] * st.m[j][2 * e] + st.m[j][2 * e + 1] * st.m[j][2 * e + 1]);
+      if (!(mx > 0.0) || !std::isfinite(mx))
+        return fail(QSV_E_INVALID, "site " + std::to_string(k) + ": Kraus operator " + std::to_string(j) + " is zero or not finite");
+      std::array<double, 8> a;
+      const double sc = 1.0 / std::sqrt(mx);
+      for (int e = 0; e < 8; ++e) a[e] = st.m[j][e] * sc;
+      kops[k].push_back(a);
+      kw[k].push_back(mx);
+    }
+  }
+  auto choose = [&](uint64_t k, double u) -> int {
+    const int nk = (int)kw[k].size();
+    if (nk == 1) return 0;
+    double acc = 0.0;
+    for (int j = 0; j < nk; ++j) {
+      acc += kw[k][j];
+      if (u < acc || j >= nk - 1) return j;
+    }
+    return nk - 1;
+  };
+  // one trajectory's gate list: clean gates of op i, then its sites' operators
+  auto trajectory_gates = [&](uint64_t t, std::vector<BGate>& out) {
+    out.clear();
+    size_t g = 0, k = 0;
+    for (uint64_t i = 0; i < n_ops; ++i) {
+      for (; g < op_end[i]; ++g) out.push_back(clean[g]);
+      for (; k < n_sites && sites[k].after_op == i; ++k) {
+        const int j = choose(k, qsv_uniform(seed, t * QSV_TRAJ_SLOTS + k));
+        emit_ctl(out, {}, n - 1 - sites[k].qubit, kops[k][j].data());
+      }
+    }
+  };
+  // the measurement of trajectory t draws qsv_uniform(seed, t * SLOTS + SLOTS - 1),
+  // which is the sampler's shot 0 of seed_m below
+  auto measure_seed = [&](uint64_t t) -> uint64_t {
+    return seed + (t * QSV_TRAJ_SLOTS + QSV_TRAJ_SLOTS - 1) * 0x9E3779B97F4A7C15ull;
+  };
+  const size_t B = precision == QSV_C128 ? 16 : 8;
+  const uint64_t S = 1ull << n;
+  const int lds_n = precision == QSV_C128 ? 13 : 14;
+  std::vector<double> host_state;
+  auto keep_state = [&](const void* dev_amps, int prec, uint64_t t, hipStream_t st) -> int {
+    // copy one final state to the caller as normalised interleaved FP64
+    host_state.resize(2 * S);
+    double* dst = final_states + t * 2 * S;
+    if (prec == QSV_C128) {
+      HIPCHK(hipMemcpyAsync(dst, dev_amps, S * 16, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+    } else {
+      std::vector<float> f(2 * S);
+      HIPCHK(hipMemcpyAsync(f.data(), dev_amps, S * 8, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      for (uint64_t e = 0; e < 2 * S; ++e) dst[e] = f[e];
+    }
+    double nrm = 0.0;
+    for (uint64_t e = 0; e < 2 * S; ++e) nrm += dst[e] * dst[e];
+    if (nrm > 0.0) {
+      const double inv = 1.0 / std::sqrt(nrm);
+      for (uint64_t e = 0; e < 2 * S; ++e) dst[e] *= inv;
+    }
+    return QSV_OK;
+  };
+  std::vector<BGate> gs;
+  if (n <= lds_n) {
+    // one workgroup per trajectory with the state in LDS, final states into
+    // an arena, one shot each; chunks bound the arena to 1 GiB
+    const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(n_traj, (1ull << 30) / (S * B)));
+    hipStream_t st = nullptr;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    void* arena = nullptr;
+    BRec* d_recs = nullptr;
+    BInst* d_insts = nullptr;
+    int32_t* d_order = nullptr;
+    SampleJob* d_jobs = nullptr;
+    uint64_t* d_out = nullptr;
+    auto cleanup = [&]() {
+      hipStreamSynchronize(st);
+      hipFree(arena);
+      hipFree(d_recs);
+      hipFree(d_insts);
+      hipFree(d_order);
+      hipFree(d_jobs);
+      hipFree(d_out);
+      hipStreamDestroy(st);
+    };
+    if (hipMalloc(&arena, chunk * S * B) != hipSuccess || hipMalloc((void**)&d_out, chunk * sizeof(uint64_t)) != hipSuccess) {
+      cleanup();
+      return fail(QSV_E_NOMEM, "trajectories: arena allocation failed");
+    }
+    if (setup_lds_batch(precision, S * B) != 0) {
+      cleanup();
+      return fail(QSV_E_DEVICE, "trajectories: LDS reservation failed");
+    }
+    for (uint64_t t0 = 0; t0 < n_traj; t0 += chunk) {
+      const uint64_t c = std::min(chunk, n_traj - t0);
+      std::vector<BRec> recs;
+      std::vector<BInst> insts(c);
+      std::vector<int32_t> order(c);
+      std::vector<SampleJob> jobs(c);
+      for (uint64_t i = 0; i < c; ++i) {
+        trajectory_gates(t0 + i, gs);
+        BInst& in = insts[i];
+        in.n = n;
+        in.g0 = recs.size();
+        for (const BGate& g : gs) {
+          BRec r{};
+          r.kind = g.kind;
+          r.a = g.a;
+          r.b = g.b;
+          r.phase_only = g.phase_only ? 1 : 0;
+          for (int cb : g.cbits) r.cmask |= 1ull << cb;
+          r.bmask = g.bmask;
+          std::memcpy(r.m, g.m, sizeof(r.m));
+          recs.push_back(r);
+        }
+        in.g1 = recs.size();
+        in.t0 = in.t1 = 0;
+        in.out_offset = i * S;
+        order[i] = (int32_t)i;
+        jobs[i] = SampleJob{i * S, i, 1, measure_seed(t0 + i), n, 0};
+      }
+      hipFree(d_recs);
+      hipFree(d_insts);
+      hipFree(d_order);
+      hipFree(d_jobs);
+      d_recs = nullptr;
+      d_insts = nullptr;
+      d_order = nullptr;
+      d_jobs = nullptr;
+      if ((rc = dev_upload(recs, &d_recs, st)) || (rc = dev_upload(insts, &d_insts, st)) ||
+          (rc = dev_upload(order, &d_order, st)) || (rc = dev_upload(jobs, &d_jobs, st))) {
+        cleanup();
+        return rc;
+      }
+      launch_lds_batch(precision, (unsigned)c, S * B, d_insts, d_recs, nullptr, nullptr, d_order, nullptr, arena, st);
+      launch_batch_sample(precision, arena, d_jobs, (unsigned)c, d_out, st);
+      if (hipMemcpyAsync(outcomes + t0, d_out, c * sizeof(uint64_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess) {
+        cleanup();
+        return fail(QSV_E_DEVICE, "trajectories: batched launch failed");
+      }
+      for (uint64_t i = 0; i < c && t0 + i < n_keep; ++i) {
+        if ((rc = keep_state((const char*)arena + i * S * B, precision, t0 + i, st))) {
+          cleanup();
+          return rc;
+        }
+      }
+    }
+    cleanup();
+    return QSV_OK;
+  }
+  // larger states: one state of this call's own, trajectory after
+  // trajectory, fused passes
+  qsv_state* s = nullptr;
+  if ((rc = qsv_state_create(n, precision, &s))) return rc;
+  uint64_t* d_out = nullptr;
+  if (hipMalloc((void**)&d_out, sizeof(uint64_t)) != hipSuccess ||
+      hipMalloc(&s->prefix, std::max<uint64_t>(1, s->S >> kSampleChunkLog) * sizeof(double)) != hipSuccess) {
+    hipFree(d_out);
+    qsv_state_destroy(s);
+    return fail(QSV_E_NOMEM, "trajectories: allocation failed");
+  }
+  for (uint64_t t = 0; t < n_traj && !rc; ++t) {
+    trajectory_gates(t, gs);
+    launch_fill_basis(s->prec, s->amps, s->S, 0, s->stream);
+    if ((rc = run_fused(s, gs, 0))) break;
+    launch_sample(s->prec, s->amps, s->S, s->prefix, 1, measure_seed(t), d_out, s->stream);
+    if (hipMemcpyAsync(outcomes + t, d_out, sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess) {
+      rc = fail(QSV_E_DEVICE, "trajectories: launch failed");
+      break;
+    }
+    if (t < n_keep) rc = keep_state(s->amps, s->prec, t, s->stream);
+  }
+  hipFree(d_out);
+  const int drc = qsv_state_destroy(s);
+  return rc ? rc : drc;
 }
 
 }  // extern "C"

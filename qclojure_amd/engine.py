@@ -296,6 +296,35 @@ def tune_fused(precision: str = "c128", num_qubits: int = 26) -> dict:
             "ms_per_layer": {int(res.candidate[i]): float(res.ms[i]) for i in range(res.n_candidates)}}
 
 
+def noisy_trajectories(num_qubits: int, ops: np.ndarray, sites, n_traj: int, seed: int,
+                       precision: str = "c128", n_keep: int = 0):
+    """qsv_noisy_trajectories: n_traj independent noisy trajectories of the
+    program `ops`, one measured outcome each, plus the first n_keep final
+    states (normalised). sites: list of {"after-op", "qubit", "kraus": [2x2]}
+    ordered by after-op."""
+    init()
+    ops = np.ascontiguousarray(ops, dtype=_abi.OP_DTYPE)
+    arr = (_abi.QsvKrausSite * max(1, len(sites)))()
+    for i, st in enumerate(sites):
+        kr = st["kraus"]
+        if not 1 <= len(kr) <= _abi.QSV_MAX_KRAUS:
+            raise ValueError(f"site {i}: 1..{_abi.QSV_MAX_KRAUS} Kraus operators expected")
+        arr[i].after_op = int(st["after-op"])
+        arr[i].qubit = int(st["qubit"])
+        arr[i].n_kraus = len(kr)
+        for j, k in enumerate(kr):
+            flat = np.asarray(k, dtype=np.complex128).reshape(4)
+            for e in range(4):
+                arr[i].m[j][2 * e] = flat[e].real
+                arr[i].m[j][2 * e + 1] = flat[e].imag
+    out = np.empty(int(n_traj), dtype=np.uint64)
+    states = np.empty((int(n_keep), 1 << num_qubits), dtype=np.complex128)
+    check(lib().qsv_noisy_trajectories(int(num_qubits), PRECISIONS[precision], _ptr(ops), len(ops),
+                                       ctypes.byref(arr), len(sites), int(n_traj), seed & (2**64 - 1),
+                                       _ptr(out), int(n_keep), _ptr(states) if n_keep else None))
+    return out, states
+
+
 def uniform(seed: int, k: int) -> float:
     """Counter-based uniform shared with the device sampler (qsv_uniform)."""
     return lib().qsv_uniform(seed & (2**64 - 1), k)
