@@ -212,6 +212,75 @@ def extract_fidelity_result(dev: DeviceState, reference_states) -> dict:
     return {"fidelities": fids, "reference-states": reference_states}
 
 
+def _eigen_probabilities(dev: DeviceState, observable, target) -> list[tuple[float, float]]:
+    """obs/measurement-probabilities (observables.clj:325-353): |<v_i|psi>|^2
+    per eigenvector of the Hermitian observable, on the device. A 2x2
+    observable with a target qubit is embedded as I..O..I: the probability of
+    eigenvalue l is <psi| I..|v><v|..I |psi> (expect_1q). Eigenvalues that
+    coincide are merged into one entry whose probability is that of the whole
+    eigenspace (the reference keys a map by eigenvalue and keeps one
+    eigenvector's overlap, which depends on its eigen-solver's basis choice)."""
+    obs = np.asarray(observable, dtype=np.complex128)
+    if obs.ndim != 2 or obs.shape[0] != obs.shape[1] or not np.allclose(obs, obs.conj().T, atol=1e-12):
+        raise ValueError("observable must be a square Hermitian matrix")
+    w, v = np.linalg.eigh(obs)
+    n = dev.num_qubits
+    probs = []
+    if obs.shape == (2, 2) and (target is not None or n == 1):
+        q = 0 if target is None else int(target)
+        for i in range(2):
+            proj = np.outer(v[:, i], v[:, i].conj())
+            probs.append(float(dev.expect_1q(q, proj).real))
+    elif obs.shape == (1 << n, 1 << n):
+        tmp = DeviceState(n, dev.precision)
+        try:
+            for i in range(len(w)):
+                tmp.upload(v[:, i])
+                probs.append(abs(tmp.inner(dev)) ** 2)
+        finally:
+            tmp.destroy()
+    else:
+        raise ValueError("observable dimension does not match the state (give a target qubit for a 2x2 observable)")
+    merged: list[list[float]] = []
+    for lam, p in zip(w, probs):
+        if merged and abs(lam - merged[-1][0]) <= 1e-12 * max(1.0, abs(lam)):
+            merged[-1][1] += p
+        else:
+            merged.append([float(lam), p])
+    return [(lam, p) for lam, p in merged]
+
+
+def extract_sample_results(dev: DeviceState, observables, shots: int = 1000, target_qubits=None,
+                           seed: int = 0) -> list:
+    """extract-sample-results (result.clj:459-494): per observable, `shots`
+    eigenvalue draws; each draw walks the (eigenvalue, probability) entries in
+    ascending eigenvalue order and takes the first whose running sum exceeds
+    the uniform, else the last entry. Draw k of observable j uses
+    qsv_uniform(seed ^ SAMPLE_SALT, j * 2^32 + k)."""
+    from .engine import uniform
+    targets = target_qubits if target_qubits is not None else [None] * len(observables)
+    out = []
+    for j, (o, t) in enumerate(zip(observables, targets)):
+        entries = _eigen_probabilities(dev, o, t)
+        outcomes = []
+        for k in range(int(shots)):
+            u = uniform(seed ^ SAMPLE_SALT, (j << 32) + k)
+            acc = 0.0
+            pick = entries[-1][0]
+            for lam, p in entries:
+                acc += p
+                if u < acc:
+                    pick = lam
+                    break
+            outcomes.append(pick)
+        out.append({"sample-outcomes": outcomes, "observable": o, "shot-count": int(shots),
+                    "target-qubits": target_qubits, "frequencies": dict(Counter(outcomes))})
+    return out
+
+
+SAMPLE_SALT = 0x53414D504C452121
+
+
 def _truthy(v) -> bool:
     """Clojure truthiness: only nil and false are false (an empty map is true)."""
     return v is not None and v is not False
@@ -245,5 +314,7 @@ def extract_results(base_result: dict, result_specs: dict, dev: DeviceState, see
     if _truthy(rs.get("fidelity")):
         out["fidelity-results"] = extract_fidelity_result(dev, rs["fidelity"]["references"])
     if _truthy(rs.get("sample")):
-        raise NotImplementedError("the :sample result type (observable eigen-sampling) is not provided")
+        spec = rs["sample"]
+        out["sample-results"] = extract_sample_results(dev, spec["observables"], int(spec.get("shots") or 1000),
+                                                       spec.get("targets"), seed)
     return out

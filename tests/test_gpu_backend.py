@@ -143,3 +143,32 @@ def test_results_same_alone_and_concurrent(sim):
     mixed = [np.asarray(be.wait_for_job(sim, j)["results"]["final-state"]["state-vector"]) for j in ids]
     for a, m in zip(alone, mixed):
         assert np.array_equal(a, m)
+
+
+def test_sample_result_type(sim):
+    """extract-sample-results (result.clj:459-494): eigenvalue draws from
+    |<v|psi>|^2, one uniform per draw walking the entries in ascending
+    eigenvalue order."""
+    from qclojure_amd import engine
+    from qclojure_amd.result import SAMPLE_SALT
+    theta = 1.1
+    Z = [[1, 0], [0, -1]]
+    X = [[0, 1], [1, 0]]
+    c = qc.add_gate(qc.create_circuit(1), "ry", target=0, angle=theta)
+    r = sim.execute(c, {"result-specs": {"sample": {"observables": [Z, X], "shots": 400}}}, 17)
+    s = r["results"]["sample-results"]
+    rs = 17 ^ 0xA5A5A5A5A5A5A5A5 ^ SAMPLE_SALT   # the result seed execute derives from the job seed
+    p_minus = np.sin(theta / 2) ** 2           # Z = -1 on |1>
+    exp = [-1.0 if engine.uniform(rs, k) < p_minus else 1.0 for k in range(400)]
+    assert s[0]["sample-outcomes"] == exp and s[0]["shot-count"] == 400
+    px_minus = (1 - np.sin(theta)) / 2          # <X> = sin(theta)
+    exp = [-1.0 if engine.uniform(rs, (1 << 32) + k) < px_minus else 1.0 for k in range(400)]
+    assert s[1]["sample-outcomes"] == exp
+    # a full-register observable with a degenerate spectrum (Z x Z) on a Bell state: always +1
+    ZZ = np.kron(Z, Z)
+    r = be.execute_circuit(sim, qc.bell_state_circuit(), {"result-specs": {"sample": {"observables": [ZZ], "shots": 50}}})
+    assert r["results"]["sample-results"][0]["frequencies"] == {1.0: 50}
+    # a 2x2 observable on a target qubit of a larger state: X on qubit 1 after H(1) -> always +1
+    c3 = qc.h_gate(qc.x_gate(qc.create_circuit(3), 0), 1)
+    r = be.execute_circuit(sim, c3, {"result-specs": {"sample": {"observables": [X], "targets": [1], "shots": 30}}})
+    assert r["results"]["sample-results"][0]["frequencies"] == {1.0: 30}
