@@ -5,6 +5,8 @@
 // and turns each into one streaming kernel on the state's own HIP stream.
 #include <algorithm>
 #include <array>
+#include <cstddef>
+#include <map>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -652,24 +654,19 @@ void end_run(std::vector<FPass>& passes)
   if (!(rn & FP_RENORM_APPLY)) rn = 0;
 }
 
-// flags: QSV_FLAG_FP64_PASSES keeps complex64 pass arithmetic in FP64.
-int run_fused(qsv_state* s, const std::vector<BGate>& gs, uint32_t flags)
+// Greedy fusion of gs (program order) into passes. emit(pass) receives each
+// pass, barrier(i) each gate that must run alone (a measurement, or a gate
+// wider than a tile); both return a status. With f32, every pass gets the
+// renormalisation flags of its position in its run of passes.
+template <class Emit, class Barrier>
+int fuse_walk(const qsv_state* s, const std::vector<BGate>& gs, bool f32, Emit&& emit, Barrier&& barrier)
 {
   const int TB = s->prec == QSV_C128 ? kTileBitsC128 : kTileBitsC64;
-  const bool f32 = s->prec == QSV_C64 && !(flags & QSV_FLAG_FP64_PASSES);
-  if (s->n < TB) {
-    for (const auto& g : gs) {
-      int rc = run_direct(s, g);
-      if (rc) return rc;
-    }
-    return QSV_OK;
-  }
   const int L = tile_low_bits(s);
   const uint64_t lowmask = (1ull << L) - 1ull;
   const uint64_t all = (s->n >= 64) ? ~0ull : ((1ull << s->n) - 1ull);
   const size_t N = gs.size();
   std::vector<char> done(N, 0);
-  std::vector<FPass> passes;
   std::vector<size_t> chosen;
   size_t first = 0;
   bool run_first = true;   // the next pass starts a run (renormalisation target)
@@ -679,9 +676,7 @@ int run_fused(qsv_state* s, const std::vector<BGate>& gs, uint32_t flags)
     if (first >= N) break;
     const BGate& g0 = gs[first];
     if (g0.kind == BG_MEASURE || __builtin_popcountll(gate_bits(g0) | lowmask) > TB) {
-      end_run(passes);
-      if ((rc = launch_passes(s, passes, f32))) return rc;
-      if ((rc = run_direct(s, g0))) return rc;
+      if ((rc = barrier(first))) return rc;
       done[first] = 1;
       run_first = true;
       continue;
@@ -711,11 +706,67 @@ int run_fused(qsv_state* s, const std::vector<BGate>& gs, uint32_t flags)
     for (size_t i = 0; i < taken; ++i) done[chosen[i]] = 1;
     p.renorm = f32 ? (FP_RENORM_OUT | (run_first ? FP_RENORM_IN : FP_RENORM_APPLY)) : 0;
     run_first = false;
-    passes.push_back(p);
-    if (passes.size() >= 512 && (rc = launch_passes(s, passes, f32))) return rc;
+    if ((rc = emit(p))) return rc;
   }
+  return QSV_OK;
+}
+
+// flags: QSV_FLAG_FP64_PASSES keeps complex64 pass arithmetic in FP64.
+int run_fused(qsv_state* s, const std::vector<BGate>& gs, uint32_t flags)
+{
+  const int TB = s->prec == QSV_C128 ? kTileBitsC128 : kTileBitsC64;
+  if (s->n < TB) {
+    for (const auto& g : gs) {
+      int rc = run_direct(s, g);
+      if (rc) return rc;
+    }
+    return QSV_OK;
+  }
+  const bool f32 = s->prec == QSV_C64 && !(flags & QSV_FLAG_FP64_PASSES);
+  std::vector<FPass> passes;
+  int rc = fuse_walk(
+      s, gs, f32,
+      [&](const FPass& p) {
+        passes.push_back(p);
+        return passes.size() >= 512 ? launch_passes(s, passes, f32) : QSV_OK;
+      },
+      [&](size_t i) {
+        end_run(passes);
+        const int r = launch_passes(s, passes, f32);
+        return r ? r : run_direct(s, gs[i]);
+      });
+  if (rc) return rc;
   end_run(passes);
   return launch_passes(s, passes, f32);
+}
+
+// Passes of one instance of a grouped batch (FP64 arithmetic, no
+// renormalisation), appended to `bytes` truncated after the last gate; false
+// if the program needs a gate run alone.
+bool plan_group_passes(int n, int prec, const std::vector<BGate>& gs, std::vector<char>& bytes,
+                       std::vector<int64_t>& offsets)
+{
+  qsv_state view;   // planning only: no device memory behind it
+  view.n = n;
+  view.prec = prec;
+  view.S = 1ull << n;
+  const size_t hdr = offsetof(FPass, gates);
+  bool ok = true;
+  fuse_walk(
+      &view, gs, false,
+      [&](const FPass& p) {
+        const size_t len = (hdr + (size_t)p.ngates * sizeof(FGate) + 255) & ~(size_t)255;
+        const size_t off = bytes.size();
+        bytes.resize(off + len, 0);
+        std::memcpy(bytes.data() + off, &p, std::min(len, sizeof(FPass)));
+        offsets.push_back((int64_t)off);
+        return QSV_OK;
+      },
+      [&](size_t) {
+        ok = false;
+        return QSV_E_UNSUPPORTED;
+      });
+  return ok;
 }
 
 int check_state(const qsv_state* s)
@@ -1156,6 +1207,22 @@ int pool_get(int n, int prec, int slot, qsv_state** out)
   return QSV_OK;
 }
 
+// Arena of grouped batch states (grow only; used under the batch mutex).
+constexpr uint64_t kGroupArenaBytes = 8ull << 30;
+void* g_group_arena = nullptr;
+uint64_t g_group_cap = 0;
+
+int grow_group_arena(uint64_t bytes)
+{
+  if (bytes <= g_group_cap) return QSV_OK;
+  if (g_group_arena) HIPCHK(hipFree(g_group_arena));
+  g_group_arena = nullptr;
+  g_group_cap = 0;
+  HIPCHK(hipMalloc(&g_group_arena, bytes));
+  g_group_cap = bytes;
+  return QSV_OK;
+}
+
 void pool_release()
 {
   std::lock_guard<std::mutex> lk(g_pool_mu);
@@ -1164,6 +1231,9 @@ void pool_release()
       for (qsv_state* s : v) qsv_state_destroy(s);
       v.clear();
     }
+  if (g_group_arena) hipFree(g_group_arena);
+  g_group_arena = nullptr;
+  g_group_cap = 0;
 }
 
 template <class T>
@@ -1305,15 +1375,107 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
     launch_lds_batch(prec, (unsigned)small_idx.size(), lds, d_insts, d_recs, d_tz, d_tc, d_order, d_energy, nullptr,
                      st);
   }
-  // circuits too large for LDS: fused passes on pooled states, spread over
-  // the pool's streams so that several run at once
-  std::vector<size_t> big_order(big_idx.size());
-  for (size_t k = 0; k < big_order.size(); ++k) big_order[k] = k;
-  std::stable_sort(big_order.begin(), big_order.end(),
-                   [&](size_t a, size_t b) { return insts[big_idx[a]].n > insts[big_idx[b]].n; });
-  int slot_rr[2][40] = {};
+  // circuits too large for LDS, grouped by qubit count: all instances of a
+  // group sit back to back in one arena and advance one pass round per launch
+  // (k_tile group mode, FP64 arithmetic); a program that needs a gate run
+  // alone takes a pooled state of its own instead
   std::vector<qsv_state*> used;
-  for (size_t k : big_order) {
+  std::vector<size_t> singles;
+  {
+    std::map<int, std::vector<size_t>, std::greater<int>> by_n;   // n (largest first) -> positions in big_idx
+    for (size_t k = 0; k < big_idx.size(); ++k) by_n[insts[big_idx[k]].n].push_back(k);
+    hipStream_t gst = nullptr;
+    char* d_bytes = nullptr;
+    int64_t* d_offs = nullptr;
+    double* d_part = nullptr;
+    auto gclean = [&]() {
+      if (gst) hipStreamSynchronize(gst);
+      hipFree(d_bytes);
+      hipFree(d_offs);
+      hipFree(d_part);
+      if (gst) hipStreamDestroy(gst);
+    };
+    if (!by_n.empty()) {
+      if (hipStreamCreateWithFlags(&gst, hipStreamNonBlocking) != hipSuccess ||
+          hipStreamWaitEvent(gst, uploaded, 0) != hipSuccess ||
+          hipMalloc((void**)&d_part, kReduceBlocks * sizeof(double)) != hipSuccess) {
+        gclean();
+        cleanup();
+        return fail(QSV_E_DEVICE, "batch: group stream setup failed");
+      }
+    }
+    for (auto& kv : by_n) {
+      const int n = kv.first;
+      const uint64_t S = 1ull << n;
+      const uint64_t per_chunk = std::max<uint64_t>(1, std::min<uint64_t>(65535, kGroupArenaBytes / (S * B)));
+      std::vector<size_t>& members = kv.second;
+      for (size_t c0 = 0; c0 < members.size(); c0 += per_chunk) {
+        // plan every instance of the chunk
+        std::vector<char> bytes;
+        std::vector<std::vector<int64_t>> offs_of;
+        std::vector<size_t> chunk;
+        for (size_t m = c0; m < members.size() && m < c0 + per_chunk; ++m) {
+          std::vector<int64_t> offs;
+          const size_t mark = bytes.size();
+          if (plan_group_passes(n, prec, big_gates[members[m]], bytes, offs)) {
+            chunk.push_back(members[m]);
+            offs_of.push_back(std::move(offs));
+          } else {
+            bytes.resize(mark);
+            singles.push_back(members[m]);
+          }
+        }
+        if (chunk.empty()) continue;
+        const uint64_t c = chunk.size();
+        size_t rounds = 0;
+        for (const auto& o : offs_of) rounds = std::max(rounds, o.size());
+        std::vector<int64_t> table(std::max<size_t>(1, rounds * c), -1);
+        for (uint64_t y = 0; y < c; ++y)
+          for (size_t r = 0; r < offs_of[y].size(); ++r) table[r * c + y] = offs_of[y][r];
+        // the previous chunk's launches read the arena and the old
+        // descriptors: wait, then grow / replace them
+        hipStreamSynchronize(gst);
+        if ((rc = grow_group_arena(c * S * B))) {
+          gclean();
+          cleanup();
+          return rc;
+        }
+        hipFree(d_bytes);
+        hipFree(d_offs);
+        d_bytes = nullptr;
+        d_offs = nullptr;
+        if (hipMalloc((void**)&d_bytes, std::max<size_t>(bytes.size(), 256)) != hipSuccess ||
+            hipMalloc((void**)&d_offs, table.size() * sizeof(int64_t)) != hipSuccess ||
+            (!bytes.empty() &&
+             hipMemcpyAsync(d_bytes, bytes.data(), bytes.size(), hipMemcpyHostToDevice, gst) != hipSuccess) ||
+            hipMemcpyAsync(d_offs, table.data(), table.size() * sizeof(int64_t), hipMemcpyHostToDevice, gst) !=
+                hipSuccess) {
+          gclean();
+          cleanup();
+          return fail(QSV_E_NOMEM, "batch: group descriptor upload failed");
+        }
+        launch_fill_group(prec, g_group_arena, S, c, gst);
+        for (size_t r = 0; r < rounds; ++r)
+          launch_tile_group(prec, g_group_arena, S, (unsigned)c, d_bytes, d_offs + r * c, gst);
+        for (uint64_t y = 0; y < c; ++y) {
+          const uint64_t i = big_idx[chunk[y]];
+          launch_zexpect(prec, (const char*)g_group_arena + y * S * B, S, d_tz + insts[i].t0, d_tc + insts[i].t0,
+                         insts[i].t1 - insts[i].t0, d_part, d_energy + i, gst);
+        }
+        // the next chunk reuses the arena and the partials
+      }
+    }
+    hipError_t ge = gst ? hipStreamSynchronize(gst) : hipSuccess;
+    gclean();
+    if (ge != hipSuccess) {
+      cleanup();
+      return fail(QSV_E_DEVICE, std::string("batch: grouped passes failed: ") + hipGetErrorString(ge));
+    }
+  }
+  // programs that need a gate run alone: fused passes on pooled states,
+  // spread over the pool's streams so that several run at once
+  int slot_rr[2][40] = {};
+  for (size_t k : singles) {
     const uint64_t i = big_idx[k];
     const int n = insts[i].n;
     qsv_state* s = nullptr;

@@ -108,19 +108,32 @@ __device__ __forceinline__ double norm2_slots(const T (&x)[kRegs])
   return s;
 }
 
-// 64 KiB of LDS per workgroup leaves room for two workgroups per CU.
-template <typename R, typename A, int TB>
+// One tile of one pass per workgroup: load, the pass's stages, store. 64 KiB
+// of LDS per workgroup leaves room for two workgroups per CU.
+// GROUP = true: many states of S amplitudes back to back, one pass round;
+// instance blockIdx.y runs the pass at byte offset offs[y] of `pass` (a
+// negative offset: no pass this round). Pass descriptors of a group are stored
+// truncated after their last gate. Group passes do not renormalise.
+template <typename R, typename A, int TB, bool GROUP>
 __global__ __launch_bounds__(1 << (TB - kRegBits)) void k_tile(typename FV<R>::T* __restrict__ psi,
-                                                               const FPass* __restrict__ pass, TileNorm nb)
+                                                               const FPass* __restrict__ pass, TileNorm nb,
+                                                               uint64_t S, const int64_t* __restrict__ offs)
 {
   using T = typename FV<R>::T;
   constexpr uint32_t FULL = (1u << TB) - 1u;
   __shared__ T lds[1 << TB];
   __shared__ double red[(1 << (TB - kRegBits)) / 64];
+  if constexpr (GROUP) {
+    const int64_t off = offs[blockIdx.y];
+    if (off < 0) return;
+    psi += (uint64_t)blockIdx.y * S;
+    pass = reinterpret_cast<const FPass*>(reinterpret_cast<const char*>(pass) + off);
+  }
+  const uint32_t tile = blockIdx.x;
 
   const uint32_t t = threadIdx.x;
   // tile base index: zeros inserted at the tile's global bits (ascending)
-  uint64_t base = blockIdx.x;
+  uint64_t base = tile;
   for (int i = 0; i < TB; ++i) {
     const int p = pass->sorted_pos[i];
     const uint64_t lo = base & ((1ull << p) - 1ull);
@@ -180,7 +193,7 @@ __global__ __launch_bounds__(1 << (TB - kRegBits)) void k_tile(typename FV<R>::T
   }
   if (rn & FP_RENORM_IN) {
     const double v = block_sum(norm2_slots(x), red);
-    if (t == 0) nb.part_in[blockIdx.x] = v;
+    if (t == 0) nb.part_in[tile] = v;
     __syncthreads();
   }
 
@@ -269,7 +282,20 @@ __global__ __launch_bounds__(1 << (TB - kRegBits)) void k_tile(typename FV<R>::T
   }
   if (rn & FP_RENORM_OUT) {
     const double v = block_sum(norm2_slots(x), red);
-    if (t == 0) nb.part_out[blockIdx.x] = v;
+    if (t == 0) nb.part_out[tile] = v;
+  }
+}
+
+// |0..0> in every state of a group.
+template <typename R>
+__global__ __launch_bounds__(256) void k_fill_group(typename FV<R>::T* __restrict__ base, uint64_t total, uint64_t S)
+{
+  using T = typename FV<R>::T;
+  for (uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x; j < total; j += (uint64_t)gridDim.x * 256) {
+    T v;
+    v.x = (j & (S - 1)) == 0 ? R(1) : R(0);
+    v.y = R(0);
+    base[j] = v;
   }
 }
 
@@ -307,14 +333,43 @@ void launch_tile(int prec, bool f32, void* psi, const FPass* dpass, uint64_t nti
                  hipStream_t st)
 {
   if (prec == QSV_C128)
-    hipLaunchKernelGGL((k_tile<double, double, kTileBitsC128>), dim3((unsigned)ntiles),
-                       dim3(1 << (kTileBitsC128 - kRegBits)), 0, st, (FV<double>::T*)psi, dpass, nb);
+    hipLaunchKernelGGL((k_tile<double, double, kTileBitsC128, false>), dim3((unsigned)ntiles),
+                       dim3(1 << (kTileBitsC128 - kRegBits)), 0, st, (FV<double>::T*)psi, dpass, nb, 0,
+                       (const int64_t*)nullptr);
   else if (f32)
-    hipLaunchKernelGGL((k_tile<float, float, kTileBitsC64>), dim3((unsigned)ntiles),
-                       dim3(1 << (kTileBitsC64 - kRegBits)), 0, st, (FV<float>::T*)psi, dpass, nb);
+    hipLaunchKernelGGL((k_tile<float, float, kTileBitsC64, false>), dim3((unsigned)ntiles),
+                       dim3(1 << (kTileBitsC64 - kRegBits)), 0, st, (FV<float>::T*)psi, dpass, nb, 0,
+                       (const int64_t*)nullptr);
   else
-    hipLaunchKernelGGL((k_tile<float, double, kTileBitsC64>), dim3((unsigned)ntiles),
-                       dim3(1 << (kTileBitsC64 - kRegBits)), 0, st, (FV<float>::T*)psi, dpass, nb);
+    hipLaunchKernelGGL((k_tile<float, double, kTileBitsC64, false>), dim3((unsigned)ntiles),
+                       dim3(1 << (kTileBitsC64 - kRegBits)), 0, st, (FV<float>::T*)psi, dpass, nb, 0,
+                       (const int64_t*)nullptr);
+}
+
+void launch_tile_group(int prec, void* base, uint64_t S, unsigned count, const char* passes, const int64_t* offs,
+                       hipStream_t st)
+{
+  // complex64 groups compute in FP64: no per-instance renormalisation needed
+  const TileNorm none{nullptr, nullptr, nullptr};
+  if (prec == QSV_C128)
+    hipLaunchKernelGGL((k_tile<double, double, kTileBitsC128, true>), dim3((unsigned)(S >> kTileBitsC128), count),
+                       dim3(1 << (kTileBitsC128 - kRegBits)), 0, st, (FV<double>::T*)base,
+                       reinterpret_cast<const FPass*>(passes), none, S, offs);
+  else
+    hipLaunchKernelGGL((k_tile<float, double, kTileBitsC64, true>), dim3((unsigned)(S >> kTileBitsC64), count),
+                       dim3(1 << (kTileBitsC64 - kRegBits)), 0, st, (FV<float>::T*)base,
+                       reinterpret_cast<const FPass*>(passes), none, S, offs);
+}
+
+void launch_fill_group(int prec, void* base, uint64_t S, uint64_t count, hipStream_t st)
+{
+  const uint64_t total = S * count;
+  const uint64_t want = (total + 255) / 256;
+  const unsigned grid = (unsigned)(want < 65536 ? want : 65536);
+  if (prec == QSV_C128)
+    hipLaunchKernelGGL((k_fill_group<double>), dim3(grid), dim3(256), 0, st, (FV<double>::T*)base, total, S);
+  else
+    hipLaunchKernelGGL((k_fill_group<float>), dim3(grid), dim3(256), 0, st, (FV<float>::T*)base, total, S);
 }
 
 void launch_norm_ctl(const TileNorm& nb, uint64_t ntiles, bool first, hipStream_t st)

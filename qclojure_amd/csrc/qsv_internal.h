@@ -147,6 +147,11 @@ struct FPass {
 void launch_tile(int prec, bool f32, void* psi, const FPass* dpass, uint64_t ntiles, const TileNorm& nb,
                  hipStream_t st);
 void launch_norm_ctl(const TileNorm& nb, uint64_t ntiles, bool first, hipStream_t st);
+// A group of `count` states of S amplitudes (S >= one tile) back to back: one
+// pass round (offs[y] = byte offset of instance y's pass in `passes`, or -1).
+void launch_tile_group(int prec, void* base, uint64_t S, unsigned count, const char* passes, const int64_t* offs,
+                       hipStream_t st);
+void launch_fill_group(int prec, void* base, uint64_t S, uint64_t count, hipStream_t st);
 
 // ---- batched small circuits (qsv_batch.hip) -----------------------------------
 constexpr int kBatchThreads = 256;

@@ -78,3 +78,22 @@ def test_batch_rejects_measurements(eng):
     ops = qc.ops_from_tuples([(OP["H"], 0, 0, 0, 0, 0.0), (OP["MEASURE"], 0, 0, 0, 1, 0.0)])
     with pytest.raises(QsvError, match="not batched"):
         eng.batch_expect_z([(2, ops, [(1.0, "ZI")])], "c64")
+
+
+@pytest.mark.parametrize("prec,tol", [("c128", 1e-10), ("c64", 1e-5)])
+def test_grouped_passes_uneven_programs(eng, oracle, prec, tol):
+    """Several instances per qubit count with programs of different lengths:
+    the grouped launches run pass rounds where some instances have no pass."""
+    from test_gpu_parity import _random_program
+    rng = np.random.default_rng(77)
+    insts = []
+    for n in (15, 15, 15, 16, 16, 17):
+        ops = _random_program(rng, n, int(rng.integers(5, 160)))
+        terms = [(float(rng.normal()), "".join(rng.choice(["I", "Z"], size=n))) for _ in range(4)]
+        insts.append((n, ops, terms))
+    print(f"grouped {prec} lengths {[len(i[1]) for i in insts]}", file=sys.stderr, flush=True)
+    got = eng.batch_expect_z(insts, prec)
+    for (n, ops, terms), g in zip(insts, got):
+        psi = oracle.run(n, ops)
+        exp = sum(c * oracle.pauli_expectation(psi, n, s) for c, s in terms)
+        assert abs(g - exp) < tol * max(1.0, sum(abs(c) for c, _ in terms)), (n, len(ops), g, exp)
