@@ -276,9 +276,9 @@ int qsv_timing_enable(qsv_state* s, int enable);
 int qsv_timing_read(qsv_state* s, qsv_kernel_stat out[QSV_KCLASS_COUNT], int reset);
 
 /* ---- fused-pass tuner ----------------------------------------------------
- * No counterpart in the reference (its simulator has no tunables). Times one
- * fused random layer (a dense 2x2 on every qubit, then CNOTs on a random
- * matching) on a scratch state of `num_qubits` (16..28; 0 = 26) for every
+ * No counterpart in the reference (its simulator has no tunables). Times a
+ * fused program of four random layers (a dense 2x2 on every qubit, then CNOTs
+ * on a random matching) on a scratch state of `num_qubits` (16..28; 0 = 26) for every
  * candidate width of the contiguous index run a tile keeps (low_bits), and
  * makes the fastest the default for later fused programs of that precision
  * in this process. QSV_TILE_LOWBITS in the environment still overrides it. */
@@ -287,7 +287,7 @@ typedef struct qsv_tune_result {
   int32_t  low_bits;               /* the chosen width                      */
   int32_t  n_candidates;
   int32_t  candidate[QSV_TUNE_MAX]; /* widths tried                         */
-  double   ms[QSV_TUNE_MAX];        /* best time of one layer for each      */
+  double   ms[QSV_TUNE_MAX];        /* best time per layer for each         */
 } qsv_tune_result;
 int qsv_tune_fused(int precision, int num_qubits, qsv_tune_result* out);
 

@@ -667,7 +667,28 @@ int fuse_walk(const qsv_state* s, const std::vector<BGate>& gs, bool f32, Emit&&
   const uint64_t all = (s->n >= 64) ? ~0ull : ((1ull << s->n) - 1ull);
   const size_t N = gs.size();
   std::vector<char> done(N, 0);
-  std::vector<size_t> chosen;
+  std::vector<uint64_t> gbits(N);
+  for (size_t i = 0; i < N; ++i) gbits[i] = gate_bits(gs[i]);
+  std::vector<size_t> chosen, win;
+  std::vector<uint64_t> cands;
+  constexpr size_t kWindow = 1024, kCandidates = 64;
+  // gates a pass with tile bits `set` could take, in program order, from the
+  // window: a gate whose bits are all in the set and that does not depend on a
+  // gate left out (one sharing a bit with it)
+  auto absorbed = [&](uint64_t set) -> int {
+    uint64_t blocked = 0;
+    int cnt = 0;
+    for (size_t i : win) {
+      const uint64_t gb = gbits[i];
+      if (!(gb & blocked) && !(gb & ~set)) {
+        if (++cnt >= kMaxPassGates) break;
+      } else {
+        blocked |= gb;
+        if ((blocked & all) == all) break;
+      }
+    }
+    return cnt;
+  };
   size_t first = 0;
   bool run_first = true;   // the next pass starts a run (renormalisation target)
   int rc;
@@ -675,13 +696,44 @@ int fuse_walk(const qsv_state* s, const std::vector<BGate>& gs, bool f32, Emit&&
     while (first < N && done[first]) ++first;
     if (first >= N) break;
     const BGate& g0 = gs[first];
-    if (g0.kind == BG_MEASURE || __builtin_popcountll(gate_bits(g0) | lowmask) > TB) {
+    if (g0.kind == BG_MEASURE || __builtin_popcountll(gbits[first] | lowmask) > TB) {
       if ((rc = barrier(first))) return rc;
       done[first] = 1;
       run_first = true;
       continue;
     }
-    uint64_t set = lowmask, blocked = 0;
+    // Tile bits: the low run and the first gate's bits, then, while there is
+    // room, the missing bits of one gate of the window at a time, choosing the
+    // addition that lets the most further gates into the pass per added bit
+    // (this pairs the partners of two-qubit gates instead of taking bits in
+    // program order).
+    win.clear();
+    for (size_t i = first; i < N && win.size() < kWindow; ++i) {
+      if (done[i]) continue;
+      if (gs[i].kind == BG_MEASURE) break;
+      win.push_back(i);
+    }
+    uint64_t set = lowmask | gbits[first];
+    while (__builtin_popcountll(set) < TB) {
+      const int base = absorbed(set);
+      uint64_t best = 0;
+      double best_score = 0.0;
+      cands.clear();
+      for (size_t k = 0; k < win.size() && k < kCandidates; ++k) {
+        const uint64_t add = gbits[win[k]] & ~set;
+        if (!add || __builtin_popcountll(set | add) > TB) continue;
+        if (std::find(cands.begin(), cands.end(), add) != cands.end()) continue;
+        cands.push_back(add);
+        const double score = double(absorbed(set | add) - base) / __builtin_popcountll(add);
+        if (score > best_score) {
+          best_score = score;
+          best = add;
+        }
+      }
+      if (!best) break;
+      set |= best;
+    }
+    uint64_t blocked = 0;
     chosen.clear();
     size_t scanned = 0;
     for (size_t i = first; i < N && scanned < 4096; ++i) {
@@ -689,7 +741,7 @@ int fuse_walk(const qsv_state* s, const std::vector<BGate>& gs, bool f32, Emit&&
       ++scanned;
       const BGate& g = gs[i];
       if (g.kind == BG_MEASURE) break;
-      const uint64_t gb = gate_bits(g);
+      const uint64_t gb = gbits[i];
       if (!(gb & blocked)) {
         const uint64_t nu = set | gb;
         if (__builtin_popcountll(nu) <= TB && chosen.size() < (size_t)kMaxPassGates) {
@@ -1212,6 +1264,27 @@ constexpr uint64_t kGroupArenaBytes = 8ull << 30;
 void* g_group_arena = nullptr;
 uint64_t g_group_cap = 0;
 
+// Grow-only scratch of a grouped batch (the stream using it is idle).
+int grow_group_scratch(uint64_t** meta, size_t* meta_cap, size_t meta_bytes, double** part, size_t* part_cap,
+                       size_t part_bytes)
+{
+  if (*meta_cap < meta_bytes) {
+    hipFree(*meta);
+    *meta = nullptr;
+    *meta_cap = 0;
+    HIPCHK(hipMalloc((void**)meta, meta_bytes));
+    *meta_cap = meta_bytes;
+  }
+  if (*part_cap < part_bytes) {
+    hipFree(*part);
+    *part = nullptr;
+    *part_cap = 0;
+    HIPCHK(hipMalloc((void**)part, part_bytes));
+    *part_cap = part_bytes;
+  }
+  return QSV_OK;
+}
+
 int grow_group_arena(uint64_t bytes)
 {
   if (bytes <= g_group_cap) return QSV_OK;
@@ -1388,17 +1461,19 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
     char* d_bytes = nullptr;
     int64_t* d_offs = nullptr;
     double* d_part = nullptr;
+    uint64_t* d_meta = nullptr;
+    size_t part_cap = 0, meta_cap = 0;
     auto gclean = [&]() {
       if (gst) hipStreamSynchronize(gst);
       hipFree(d_bytes);
       hipFree(d_offs);
       hipFree(d_part);
+      hipFree(d_meta);
       if (gst) hipStreamDestroy(gst);
     };
     if (!by_n.empty()) {
       if (hipStreamCreateWithFlags(&gst, hipStreamNonBlocking) != hipSuccess ||
-          hipStreamWaitEvent(gst, uploaded, 0) != hipSuccess ||
-          hipMalloc((void**)&d_part, kReduceBlocks * sizeof(double)) != hipSuccess) {
+          hipStreamWaitEvent(gst, uploaded, 0) != hipSuccess) {
         gclean();
         cleanup();
         return fail(QSV_E_DEVICE, "batch: group stream setup failed");
@@ -1457,12 +1532,26 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
         launch_fill_group(prec, g_group_arena, S, c, gst);
         for (size_t r = 0; r < rounds; ++r)
           launch_tile_group(prec, g_group_arena, S, (unsigned)c, d_bytes, d_offs + r * c, gst);
+        // energies: one launch for the chunk
+        std::vector<uint64_t> meta(3 * c);   // term ranges, then destinations
         for (uint64_t y = 0; y < c; ++y) {
           const uint64_t i = big_idx[chunk[y]];
-          launch_zexpect(prec, (const char*)g_group_arena + y * S * B, S, d_tz + insts[i].t0, d_tc + insts[i].t0,
-                         insts[i].t1 - insts[i].t0, d_part, d_energy + i, gst);
+          meta[2 * y] = insts[i].t0;
+          meta[2 * y + 1] = insts[i].t1;
+          meta[2 * c + y] = i;
         }
-        // the next chunk reuses the arena and the partials
+        const unsigned bps = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(64, S / 4096));
+        if ((rc = grow_group_scratch(&d_meta, &meta_cap, meta.size() * sizeof(uint64_t), &d_part, &part_cap,
+                                     (size_t)c * bps * sizeof(double))) ||
+            hipMemcpyAsync(d_meta, meta.data(), meta.size() * sizeof(uint64_t), hipMemcpyHostToDevice, gst) !=
+                hipSuccess) {
+          gclean();
+          cleanup();
+          return rc ? rc : fail(QSV_E_DEVICE, "batch: energy metadata upload failed");
+        }
+        launch_zexpect_group(prec, g_group_arena, S, (unsigned)c, d_tz, d_tc, d_meta, d_part, bps, d_meta + 2 * c,
+                             d_energy, gst);
+        // the next chunk reuses the arena, the partials and the metadata
       }
     }
     hipError_t ge = gst ? hipStreamSynchronize(gst) : hipSuccess;
@@ -2275,21 +2364,24 @@ int qsv_tune_fused(int precision, int num_qubits, qsv_tune_result* out)
   qsv_state* s = nullptr;
   int rc = qsv_state_create(n, precision, &s);
   if (rc) return rc;
-  // the timed layer: the dense unitary RZ(0.7) RY(0.3) on every bit, then
-  // CNOTs on a fixed pseudo-random matching
+  // the timed program: kTuneLayers layers of the dense unitary RZ(0.7) RY(0.3)
+  // on every bit, then CNOTs on a fixed pseudo-random matching per layer
+  constexpr int kTuneLayers = 4;
   std::vector<BGate> layer;
   const double c = std::cos(0.15), sn = std::sin(0.15), zc = std::cos(0.35), zs = std::sin(0.35);
   const double m[8] = {c * zc, -c * zs, -sn * zc, sn * zs, sn * zc, sn * zs, c * zc, c * zs};
-  std::vector<int> perm(n);
-  for (int b = 0; b < n; ++b) perm[b] = b;
-  uint64_t x = 0x2545F4914F6CDD1Dull;
-  for (int b = n - 1; b > 0; --b) {
-    x ^= x << 13; x ^= x >> 7; x ^= x << 17;
-    std::swap(perm[b], perm[(int)(x % (uint64_t)(b + 1))]);
-  }
-  for (int b = 0; b < n; ++b) emit_ctl(layer, {}, b, m);
   const double xm[8] = {0, 0, 1, 0, 1, 0, 0, 0};
-  for (int k = 0; k + 1 < n; k += 2) emit_ctl(layer, {perm[k]}, perm[k + 1], xm);
+  std::vector<int> perm(n);
+  uint64_t x = 0x2545F4914F6CDD1Dull;
+  for (int l = 0; l < kTuneLayers; ++l) {
+    for (int b = 0; b < n; ++b) perm[b] = b;
+    for (int b = n - 1; b > 0; --b) {
+      x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+      std::swap(perm[b], perm[(int)(x % (uint64_t)(b + 1))]);
+    }
+    for (int b = 0; b < n; ++b) emit_ctl(layer, {}, b, m);
+    for (int k = 0; k + 1 < n; k += 2) emit_ctl(layer, {perm[k]}, perm[k + 1], xm);
+  }
   const int TB = precision == QSV_C128 ? kTileBitsC128 : kTileBitsC64;
   const int lo = low_bit_threshold(s), hi = std::min(TB - 4, lo + QSV_TUNE_MAX - 1);
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -2310,7 +2402,7 @@ int qsv_tune_fused(int precision, int num_qubits, qsv_tune_result* out)
           rc = fail(QSV_E_DEVICE, "tuning timing failed");
           break;
         }
-        if (rep) t = std::min(t, (double)ms);
+        if (rep) t = std::min(t, (double)ms / kTuneLayers);
       }
       if (rc) break;
       out->candidate[out->n_candidates] = L;

@@ -226,6 +226,61 @@ __global__ __launch_bounds__(256) void k_sum_to(const double* __restrict__ parti
   if (threadIdx.x == 0) *out = red[0] + red[1] + red[2] + red[3];
 }
 
+// <H> = sum_k c_k <Z-string_k> of `count` states of S amplitudes back to
+// back: block (x, y) sums its share of state y into partials[y * gridDim.x + x].
+template <typename R>
+__global__ __launch_bounds__(256) void k_zexpect_group(const typename BV<R>::T* __restrict__ base, uint64_t S,
+                                                       const uint64_t* __restrict__ tz,
+                                                       const double* __restrict__ tc,
+                                                       const uint64_t* __restrict__ trange,
+                                                       double* __restrict__ partials)
+{
+  __shared__ double red[4];
+  const uint64_t y = blockIdx.y;
+  const auto* psi = base + y * S;
+  const uint64_t t0 = trange[2 * y], t1 = trange[2 * y + 1];
+  double acc = 0.0;
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x; j < S; j += stride) {
+    const auto a = psi[j];
+    const double p = (double)a.x * a.x + (double)a.y * a.y;
+    double h = 0.0;
+    for (uint64_t t = t0; t < t1; ++t) h += (__popcll(j & tz[t]) & 1) ? -tc[t] : tc[t];
+    acc += p * h;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partials[y * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// out[dest[y]] = fixed-order sum of the nb partials of state y.
+__global__ __launch_bounds__(64) void k_sum_group(const double* __restrict__ partials, uint32_t nb,
+                                                  const uint64_t* __restrict__ dest, double* __restrict__ out)
+{
+  const uint64_t y = blockIdx.x;
+  double acc = 0.0;
+  for (uint32_t k = threadIdx.x; k < nb; k += 64) acc += partials[y * nb + k];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if (threadIdx.x == 0) out[dest[y]] = acc;
+}
+
+void launch_zexpect_group(int prec, const void* base, uint64_t S, unsigned count, const uint64_t* tz,
+                          const double* tc, const uint64_t* trange, double* partials, unsigned blocks_per_state,
+                          const uint64_t* dest, double* out, hipStream_t st)
+{
+  if (count == 0) return;
+  if (prec == QSV_C128)
+    hipLaunchKernelGGL((k_zexpect_group<double>), dim3(blocks_per_state, count), dim3(256), 0, st,
+                       (const BV<double>::T*)base, S, tz, tc, trange, partials);
+  else
+    hipLaunchKernelGGL((k_zexpect_group<float>), dim3(blocks_per_state, count), dim3(256), 0, st,
+                       (const BV<float>::T*)base, S, tz, tc, trange, partials);
+  hipLaunchKernelGGL(k_sum_group, dim3(count), dim3(64), 0, st, partials, blocks_per_state, dest, out);
+}
+
 int setup_lds_batch(int prec, size_t lds_bytes)
 {
   hipError_t e;

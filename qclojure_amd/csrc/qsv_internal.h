@@ -180,6 +180,12 @@ void launch_lds_batch(int prec, unsigned nblocks, size_t lds_bytes, const BInst*
                       void* out_states, hipStream_t st);
 void launch_zexpect(int prec, const void* psi, uint64_t S, const uint64_t* tz, const double* tc, uint64_t nterms,
                     double* partials, double* out, hipStream_t st);
+// Energies of `count` states of S amplitudes back to back: trange[2y..2y+1]
+// = term range of state y, partials >= count * blocks_per_state doubles,
+// out[dest[y]] = <H> of state y.
+void launch_zexpect_group(int prec, const void* base, uint64_t S, unsigned count, const uint64_t* tz,
+                          const double* tc, const uint64_t* trange, double* partials, unsigned blocks_per_state,
+                          const uint64_t* dest, double* out, hipStream_t st);
 
 // Shots for many small states stored back to back in an arena.
 struct SampleJob {

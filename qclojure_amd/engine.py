@@ -193,6 +193,71 @@ class DeviceState:
                 for i, name in enumerate(_abi.KCLASS_NAMES)}
 
 
+class PackedBatch:
+    """Instances flattened into the arrays of a qsv_batch_desc. An instance's
+    terms are either a list of (coefficient, I/Z string) or a pair of arrays
+    (Z masks with bit q = Z on qubit q, coefficients)."""
+
+    def __init__(self, instances, precision="c64"):
+        nq, opoff, toff, zm, co, all_ops = [], [0], [0], [], [], []
+        zparts, cparts = [], []
+        masks: dict[str, int] = {}
+        for (n, ops, terms) in instances:
+            nq.append(int(n))
+            ops = np.ascontiguousarray(ops, dtype=_abi.OP_DTYPE)
+            all_ops.append(ops)
+            opoff.append(opoff[-1] + len(ops))
+            if isinstance(terms, tuple) and len(terms) == 2 and isinstance(terms[0], np.ndarray):
+                zmask = np.asarray(terms[0], dtype=np.uint64)
+                if zm:
+                    zparts.append(np.asarray(zm, dtype=np.uint64))
+                    cparts.append(np.asarray(co, dtype=np.float64))
+                    zm, co = [], []
+                zparts.append(zmask)
+                cparts.append(np.asarray(terms[1], dtype=np.float64))
+                toff.append(toff[-1] + len(zmask))
+                continue
+            start = len(zm)
+            for (c, s) in terms:
+                m = masks.get(s)
+                if m is None:
+                    if len(s) != n or set(s) - {"I", "Z"}:
+                        raise ValueError(f"batched Hamiltonians take I/Z strings of length {n}: {s!r}")
+                    m = masks[s] = sum(1 << q for q, ch in enumerate(s) if ch == "Z")
+                elif len(s) != n:
+                    raise ValueError(f"batched Hamiltonians take I/Z strings of length {n}: {s!r}")
+                zm.append(m)
+                co.append(float(c))
+            toff.append(toff[-1] + len(zm) - start)
+        if zm:
+            zparts.append(np.asarray(zm, dtype=np.uint64))
+            cparts.append(np.asarray(co, dtype=np.float64))
+        self.n = len(nq)
+        self.precision = PRECISIONS[precision]
+        self._nq = np.asarray(nq, dtype=np.int32)
+        self._opoff = np.asarray(opoff, dtype=np.uint64)
+        self._toff = np.asarray(toff, dtype=np.uint64)
+        self._zm = np.concatenate(zparts) if zparts else np.zeros(1, dtype=np.uint64)
+        self._co = np.concatenate(cparts) if cparts else np.zeros(1, dtype=np.float64)
+        # one byte copy per instance into the flat record array
+        flat = np.zeros(max(1, opoff[-1]), dtype=_abi.OP_DTYPE)
+        raw = flat.view(np.uint8)
+        rec = _abi.OP_DTYPE.itemsize
+        for k, ops in enumerate(all_ops):
+            if len(ops):
+                raw[opoff[k] * rec:opoff[k + 1] * rec] = ops.view(np.uint8)
+        self._ops = flat
+
+    def run(self) -> np.ndarray:
+        """qsv_batch_expect_z: <H> of every instance."""
+        init()
+        desc = _abi.QsvBatchDesc(self.n, self._nq.ctypes.data, self._opoff.ctypes.data, self._ops.ctypes.data,
+                                 self._toff.ctypes.data, self._zm.ctypes.data, self._co.ctypes.data, self.precision)
+        out = np.zeros(self.n, dtype=np.float64)
+        check(lib().qsv_batch_expect_z(ctypes.byref(desc), _ptr(out)))
+        return out
+
+
 def batch_expect_z(instances, precision="c64") -> np.ndarray:
     """Run many independent circuits from |0..0> and return each one's
     <H> = sum_k c_k <Z-string_k> (qsv_batch_expect_z).
@@ -200,31 +265,7 @@ def batch_expect_z(instances, precision="c64") -> np.ndarray:
     instances: iterable of (num_qubits, ops (qsv_op records), terms) where terms
     is a list of (coefficient, pauli_string) with only I/Z characters.
     """
-    init()
-    nq, opoff, toff, zm, co, all_ops = [], [0], [0], [], [], []
-    for (n, ops, terms) in instances:
-        nq.append(int(n))
-        ops = np.ascontiguousarray(ops, dtype=_abi.OP_DTYPE)
-        all_ops.append(ops)
-        opoff.append(opoff[-1] + len(ops))
-        for (c, s) in terms:
-            if len(s) != n or set(s) - {"I", "Z"}:
-                raise ValueError(f"batched Hamiltonians take I/Z strings of length {n}: {s!r}")
-            zm.append(sum(1 << q for q, ch in enumerate(s) if ch == "Z"))
-            co.append(float(c))
-        toff.append(len(zm))
-    N = len(nq)
-    nq_a = np.asarray(nq, dtype=np.int32)
-    opoff_a = np.asarray(opoff, dtype=np.uint64)
-    toff_a = np.asarray(toff, dtype=np.uint64)
-    zm_a = np.asarray(zm if zm else [0], dtype=np.uint64)
-    co_a = np.asarray(co if co else [0.0], dtype=np.float64)
-    ops_a = np.ascontiguousarray(np.concatenate(all_ops) if all_ops else np.zeros(1, _abi.OP_DTYPE))
-    desc = _abi.QsvBatchDesc(N, nq_a.ctypes.data, opoff_a.ctypes.data, ops_a.ctypes.data, toff_a.ctypes.data,
-                             zm_a.ctypes.data, co_a.ctypes.data, PRECISIONS[precision])
-    out = np.zeros(N, dtype=np.float64)
-    check(lib().qsv_batch_expect_z(ctypes.byref(desc), _ptr(out)))
-    return out
+    return PackedBatch(instances, precision).run()
 
 
 class Server:
@@ -287,7 +328,7 @@ class Server:
 
 
 def tune_fused(precision: str = "c128", num_qubits: int = 26) -> dict:
-    """Time one fused random layer for every candidate tile low-run width and
+    """Time a fused four-layer random program for every candidate tile low-run width and
     make the fastest the default for `precision` (qsv_tune_fused)."""
     init()
     res = _abi.QsvTuneResult()

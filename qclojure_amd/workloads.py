@@ -95,6 +95,22 @@ def qaoa_ops(inst: dict) -> np.ndarray:
     return out
 
 
+def qaoa_terms(inst: dict) -> tuple[np.ndarray, np.ndarray]:
+    """The instance's MaxCut Hamiltonian as (Z masks, coefficients), in term
+    order; bit q of a mask is Z on qubit q."""
+    hp = inst["hamiltonian"]
+    masks = np.zeros(len(hp), dtype=np.uint64)
+    coeffs = np.zeros(len(hp), dtype=np.float64)
+    for k, t in enumerate(hp):
+        m = 0
+        for q, ch in enumerate(t["pauli-string"]):
+            if ch == "Z":
+                m |= 1 << q
+        masks[k] = m
+        coeffs[k] = t["coefficient"]
+    return masks, coeffs
+
+
 def layer_slices(n: int, layers: int) -> list[slice]:
     per = n + n // 2
     return [slice(k * per, (k + 1) * per) for k in range(layers)]
