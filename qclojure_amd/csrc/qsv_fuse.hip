@@ -1,7 +1,8 @@
 // qsv_fuse.hip — fused tiled passes: many gates per read/write of the state.
 //
-// A pass owns a set of TB "tile" bits (12 for complex128, 13 for complex64,
-// 64 KiB of LDS either way). The 2^(n-TB) tiles are independent: a workgroup
+// A pass owns a set of TB = 12 "tile" bits (64 KiB of LDS for complex128, two
+// workgroups per CU; 32 KiB for complex64, three per CU, which their register
+// count allows). The 2^(n-TB) tiles are independent: a workgroup
 // of 2^(TB-4) threads loads one tile, each thread holding 16 amplitudes in
 // registers, applies the pass's gates and stores it back, so the pass moves
 // the state through HBM once however many gates it holds.
@@ -71,16 +72,76 @@ __device__ __forceinline__ T cscale(A re, A im, T a)
   return y;
 }
 
-// Dense 2x2 on register bit RBIT; slot k acts when (k & kc) == kc.
-template <int RBIT, typename T, typename R, typename A>
+// Real matrix (DM_REAL): each output is a real combination of a and b.
+template <typename T, typename R, typename A>
+__device__ __forceinline__ void rmad_pair(const M8<A>& m, T& a, T& b)
+{
+  const A ax = a.x, ay = a.y, bx = b.x, by = b.y;
+  T y0, y1;
+  y0.x = (R)(m.m0 * ax + m.m2 * bx);
+  y0.y = (R)(m.m0 * ay + m.m2 * by);
+  y1.x = (R)(m.m4 * ax + m.m6 * bx);
+  y1.y = (R)(m.m4 * ay + m.m6 * by);
+  a = y0;
+  b = y1;
+}
+
+// Real diagonal, imaginary off-diagonal (DM_RX): m01 = i m3, m10 = i m5.
+template <typename T, typename R, typename A>
+__device__ __forceinline__ void xmad_pair(const M8<A>& m, T& a, T& b)
+{
+  const A ax = a.x, ay = a.y, bx = b.x, by = b.y;
+  T y0, y1;
+  y0.x = (R)(m.m0 * ax - m.m3 * by);
+  y0.y = (R)(m.m0 * ay + m.m3 * bx);
+  y1.x = (R)(m.m6 * bx - m.m5 * ay);
+  y1.y = (R)(m.m6 * by + m.m5 * ax);
+  a = y0;
+  b = y1;
+}
+
+// Dense 2x2 on register bit RBIT; slot k acts when (k & kc) == kc (ALL: no
+// register-bit controls, every slot acts, and no per-slot selects are
+// generated). DM is the matrix structure (DM_*), known for the whole gate.
+template <int RBIT, int DM, bool ALL, typename T, typename R, typename A>
 __device__ __forceinline__ void reg_dense(T (&x)[kRegs], const M8<A>& m, uint32_t kc)
 {
 #pragma unroll
   for (int k = 0; k < kRegs; ++k) {
     if (!(k & (1 << RBIT))) {
-      if ((k & kc) == kc) cmad_pair<T, R, A>(m, x[k], x[k | (1 << RBIT)]);
+      if (ALL || (k & kc) == kc) {
+        T& a = x[k];
+        T& b = x[k | (1 << RBIT)];
+        if constexpr (DM == DM_EXCHANGE) {
+          const T t = a;
+          a = b;
+          b = t;
+        } else if constexpr (DM == DM_REAL) {
+          rmad_pair<T, R, A>(m, a, b);
+        } else if constexpr (DM == DM_RX) {
+          xmad_pair<T, R, A>(m, a, b);
+        } else {
+          cmad_pair<T, R, A>(m, a, b);
+        }
+      }
     }
   }
+}
+
+template <int RBIT, bool ALL, typename T, typename R, typename A>
+__device__ __forceinline__ void reg_dense_dm(T (&x)[kRegs], const M8<A>& m, uint32_t kc, int dm)
+{
+  if (dm == DM_EXCHANGE) reg_dense<RBIT, DM_EXCHANGE, ALL, T, R, A>(x, m, kc);
+  else if (dm == DM_REAL) reg_dense<RBIT, DM_REAL, ALL, T, R, A>(x, m, kc);
+  else if (dm == DM_RX) reg_dense<RBIT, DM_RX, ALL, T, R, A>(x, m, kc);
+  else reg_dense<RBIT, DM_GENERAL, ALL, T, R, A>(x, m, kc);
+}
+
+template <int RBIT, typename T, typename R, typename A>
+__device__ __forceinline__ void reg_dense_any(T (&x)[kRegs], const M8<A>& m, uint32_t kc, int dm)
+{
+  if (kc == 0) reg_dense_dm<RBIT, true, T, R, A>(x, m, kc, dm);
+  else reg_dense_dm<RBIT, false, T, R, A>(x, m, kc, dm);
 }
 
 // Sum of v over the workgroup in a fixed order (valid in thread 0).
@@ -108,8 +169,7 @@ __device__ __forceinline__ double norm2_slots(const T (&x)[kRegs])
   return s;
 }
 
-// One tile of one pass per workgroup: load, the pass's stages, store. 64 KiB
-// of LDS per workgroup leaves room for two workgroups per CU.
+// One tile of one pass per workgroup: load, the pass's stages, store.
 // GROUP = true: many states of S amplitudes back to back, one pass round;
 // instance blockIdx.y runs the pass at byte offset offs[y] of `pass` (a
 // negative offset: no pass this round). Pass descriptors of a group are stored
@@ -224,14 +284,15 @@ __global__ __launch_bounds__(1 << (TB - kRegBits)) void k_tile(typename FV<R>::T
         const M8<A> m{(A)g->m[0], (A)g->m[1], (A)g->m[2], (A)g->m[3],
                       (A)g->m[4], (A)g->m[5], (A)g->m[6], (A)g->m[7]};
         const uint32_t tb = 1u << g->a;
-        if (tb == r0) reg_dense<0, T, R, A>(x, m, kc);
-        else if (tb == r1) reg_dense<1, T, R, A>(x, m, kc);
-        else if (tb == r2) reg_dense<2, T, R, A>(x, m, kc);
-        else reg_dense<3, T, R, A>(x, m, kc);
+        const int dm = g->sub;
+        if (tb == r0) reg_dense_any<0, T, R, A>(x, m, kc, dm);
+        else if (tb == r1) reg_dense_any<1, T, R, A>(x, m, kc, dm);
+        else if (tb == r2) reg_dense_any<2, T, R, A>(x, m, kc, dm);
+        else reg_dense_any<3, T, R, A>(x, m, kc, dm);
       } else if (kind == FG_PHASE) {
         const uint32_t tb = 1u << g->a;
         const A d0r = (A)g->m[0], d0i = (A)g->m[1], d1r = (A)g->m[6], d1i = (A)g->m[7];
-        const bool skip0 = g->phase_only != 0;
+        const bool skip0 = g->sub != 0;
 #pragma unroll
         for (int k = 0; k < kRegs; ++k) {
           const uint32_t l = kl(k);

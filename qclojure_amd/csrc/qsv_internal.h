@@ -96,11 +96,15 @@ void launch_collapse(int prec, void* psi, uint64_t S, int m, const int* pos, uin
 
 // ---- fused tiled passes (qsv_fuse.hip) ---------------------------------------
 constexpr int kTileBitsC128 = 12;   // 4096 x 16 B = 64 KiB of LDS per tile
-constexpr int kTileBitsC64 = 13;    // 8192 x  8 B = 64 KiB
+constexpr int kTileBitsC64 = 12;    // 4096 x  8 B = 32 KiB: three workgroups per CU
 constexpr int kMaxPassGates = 96;
 constexpr int kMaxPassStages = 32;
 
 enum { FG_DENSE = 0, FG_PHASE = 1, FG_SWAP = 2, FG_BLOCKADE = 3 };
+// Structure of a dense 2x2, which decides the arithmetic it needs per pair:
+// general (16 real multiply-adds), real entries (8), real diagonal with
+// imaginary off-diagonal, as RX (8), or the exchange [[0,1],[1,0]] (none).
+enum { DM_GENERAL = 0, DM_REAL = 1, DM_RX = 2, DM_EXCHANGE = 3 };
 
 // A gate in tile-local bit coordinates.
 struct FGate {
@@ -108,7 +112,7 @@ struct FGate {
   int32_t a, b;        // dense/phase target a; swap bits a, b
   uint32_t cmask;      // local control bits (all must be 1)
   uint32_t bmask;      // blockade bits
-  int32_t phase_only;  // FG_PHASE with d0 == 1
+  int32_t sub;         // FG_PHASE: 1 when d0 == 1; FG_DENSE: DM_* structure of m
   double m[8];         // dense: row-major 2x2; phase: d0 = m[0..1], d1 = m[6..7]; swap/blockade: phase m[0..1]
 };
 
