@@ -137,6 +137,30 @@ __device__ __forceinline__ void reg_dense_dm(T (&x)[kRegs], const M8<A>& m, uint
   else reg_dense<RBIT, DM_GENERAL, ALL, T, R, A>(x, m, kc);
 }
 
+// Diagonal diag(d0, d1) on register bit RBIT (skip0: d0 == 1); slot k acts
+// when (k & kc) == kc, every slot when ALL.
+template <int RBIT, bool ALL, typename T, typename R, typename A>
+__device__ __forceinline__ void reg_phase(T (&x)[kRegs], uint32_t kc, A d0r, A d0i, A d1r, A d1i, bool skip0)
+{
+#pragma unroll
+  for (int k = 0; k < kRegs; ++k) {
+    if (ALL || (k & kc) == kc) {
+      if (k & (1 << RBIT)) x[k] = cscale<T, R, A>(d1r, d1i, x[k]);
+      else if (!skip0) x[k] = cscale<T, R, A>(d0r, d0i, x[k]);
+    }
+  }
+}
+
+template <bool ALL, typename T, typename R, typename A>
+__device__ __forceinline__ void reg_phase_any(T (&x)[kRegs], int rb, uint32_t kc, A d0r, A d0i, A d1r, A d1i,
+                                              bool skip0)
+{
+  if (rb == 0) reg_phase<0, ALL, T, R, A>(x, kc, d0r, d0i, d1r, d1i, skip0);
+  else if (rb == 1) reg_phase<1, ALL, T, R, A>(x, kc, d0r, d0i, d1r, d1i, skip0);
+  else if (rb == 2) reg_phase<2, ALL, T, R, A>(x, kc, d0r, d0i, d1r, d1i, skip0);
+  else reg_phase<3, ALL, T, R, A>(x, kc, d0r, d0i, d1r, d1i, skip0);
+}
+
 template <int RBIT, typename T, typename R, typename A>
 __device__ __forceinline__ void reg_dense_any(T (&x)[kRegs], const M8<A>& m, uint32_t kc, int dm)
 {
@@ -175,7 +199,7 @@ __device__ __forceinline__ double norm2_slots(const T (&x)[kRegs])
 // negative offset: no pass this round). Pass descriptors of a group are stored
 // truncated after their last gate. Group passes do not renormalise.
 template <typename R, typename A, int TB, bool GROUP>
-__global__ __launch_bounds__(1 << (TB - kRegBits)) void k_tile(typename FV<R>::T* __restrict__ psi,
+__global__ __launch_bounds__(1 << (TB - kRegBits), 3) void k_tile(typename FV<R>::T* __restrict__ psi,
                                                                const FPass* __restrict__ pass, TileNorm nb,
                                                                uint64_t S, const int64_t* __restrict__ offs)
 {
@@ -290,15 +314,30 @@ __global__ __launch_bounds__(1 << (TB - kRegBits)) void k_tile(typename FV<R>::T
         else if (tb == r2) reg_dense_any<2, T, R, A>(x, m, kc, dm);
         else reg_dense_any<3, T, R, A>(x, m, kc, dm);
       } else if (kind == FG_PHASE) {
+        // controls and target split like a dense gate's: thread-index bits
+        // decide once per thread, register bits per slot at compile time
         const uint32_t tb = 1u << g->a;
+        const uint32_t cthr = cm & ~reg;
+        if ((tl & cthr) != cthr) continue;
+        const uint32_t kc = ((cm & r0) ? 1u : 0u) | ((cm & r1) ? 2u : 0u) | ((cm & r2) ? 4u : 0u) |
+                            ((cm & r3) ? 8u : 0u);
         const A d0r = (A)g->m[0], d0i = (A)g->m[1], d1r = (A)g->m[6], d1i = (A)g->m[7];
         const bool skip0 = g->sub != 0;
+        if (tb & reg) {
+          const int rb = (tb == r0) ? 0 : (tb == r1) ? 1 : (tb == r2) ? 2 : 3;
+          if (kc == 0) reg_phase_any<true, T, R, A>(x, rb, 0u, d0r, d0i, d1r, d1i, skip0);
+          else reg_phase_any<false, T, R, A>(x, rb, kc, d0r, d0i, d1r, d1i, skip0);
+        } else {
+          const bool hi = (tl & tb) != 0;
+          if (!hi && skip0) continue;
+          const A dr = hi ? d1r : d0r, di = hi ? d1i : d0i;
+          if (kc == 0) {
 #pragma unroll
-        for (int k = 0; k < kRegs; ++k) {
-          const uint32_t l = kl(k);
-          if ((l & cm) == cm) {
-            if (l & tb) x[k] = cscale<T, R, A>(d1r, d1i, x[k]);
-            else if (!skip0) x[k] = cscale<T, R, A>(d0r, d0i, x[k]);
+            for (int k = 0; k < kRegs; ++k) x[k] = cscale<T, R, A>(dr, di, x[k]);
+          } else {
+#pragma unroll
+            for (int k = 0; k < kRegs; ++k)
+              if ((k & kc) == kc) x[k] = cscale<T, R, A>(dr, di, x[k]);
           }
         }
       } else if (kind == FG_SWAP) {
