@@ -291,6 +291,73 @@ typedef struct qsv_tune_result {
 } qsv_tune_result;
 int qsv_tune_fused(int precision, int num_qubits, qsv_tune_result* out);
 
+/* ---- complex linear algebra on the device: the :hip math backend -----------
+ * The MatrixAlgebra protocol and eigen-hermitian of
+ * src/org/soulspace/qclojure/domain/math/protocols.clj:81-375, as implemented
+ * by the reference's backends (math/fastmath/complex_linear_algebra.clj,
+ * math/clojure/complex_linear_algebra.clj). A qsv_mat is a rows x cols
+ * complex128 matrix in HBM, row-major, (re, im) interleaved; vectors are
+ * n x 1. Results go to caller-created matrices of the right shape; calls are
+ * synchronous and serialised on one library stream. */
+typedef struct qsv_mat qsv_mat;
+/* matrix->backend / vector->backend (protocols.clj:18-56): a zero-filled matrix,
+ * then its contents from host memory; backend->matrix: the contents back. */
+int qsv_mat_create(int64_t rows, int64_t cols, qsv_mat** out);
+int qsv_mat_destroy(qsv_mat* m);
+int qsv_mat_upload(qsv_mat* m, const double* host);        /* rows*cols (re, im) pairs */
+int qsv_mat_download(const qsv_mat* m, double* host);
+int qsv_mat_copy(const qsv_mat* src, qsv_mat* dst);        /* same element count */
+/* shape (protocols.clj:106-114). */
+int qsv_mat_shape(const qsv_mat* m, int64_t* rows, int64_t* cols);
+/* A complex128 state's amplitudes as a 2^n x 1 matrix (no copy; the view must
+ * not outlive the state). No counterpart in the reference: it lets results of
+ * the engine enter the backend without a host round trip. */
+int qsv_mat_view_state(qsv_state* s, qsv_mat** out);
+/* add, subtract, scale, negate (protocols.clj:116-160): C = alpha A + beta B
+ * (B may be NULL: C = alpha A). */
+int qsv_la_axpby(double alpha_re, double alpha_im, const qsv_mat* A,
+                 double beta_re, double beta_im, const qsv_mat* B, qsv_mat* C);
+/* hadamard-product (protocols.clj:213-222). */
+int qsv_la_hadamard(const qsv_mat* A, const qsv_mat* B, qsv_mat* C);
+/* matrix-multiply (protocols.clj:163-172): C = op(A) B, op(A) = A or, with
+ * conj_transpose_a, A^H (used by unitary?, protocols.clj:328-337). */
+int qsv_la_gemm(int conj_transpose_a, const qsv_mat* A, const qsv_mat* B, qsv_mat* C);
+/* matrix-vector-product (protocols.clj:174-183). */
+int qsv_la_gemv(const qsv_mat* A, const qsv_mat* x, qsv_mat* y);
+/* inner-product (protocols.clj:185-197): sum conj(x_i) y_i with conj_x = 1. */
+int qsv_la_dot(int conj_x, const qsv_mat* x, const qsv_mat* y, double out[2]);
+/* norm2 (protocols.clj:273-282): sqrt(sum |x_i|^2). */
+int qsv_la_norm2(const qsv_mat* x, double* out);
+/* outer-product (protocols.clj:199-211): C_ij = x_i conj(y_j). */
+int qsv_la_outer(const qsv_mat* x, const qsv_mat* y, qsv_mat* C);
+/* kronecker-product (protocols.clj:224-237). */
+int qsv_la_kron(const qsv_mat* A, const qsv_mat* B, qsv_mat* C);
+/* transpose / conjugate-transpose (protocols.clj:239-261). */
+int qsv_la_transpose(int conj, const qsv_mat* A, qsv_mat* C);
+/* trace (protocols.clj:263-271). */
+int qsv_la_trace(const qsv_mat* A, double out[2]);
+/* hermitian?, diagonal?, unitary? (protocols.clj:306-337): the largest
+ * element-wise deviation, which the caller compares with its tolerance:
+ * max |A_ij - conj(A_ji)|, max_{i != j} |A_ij|, max |A_ij - delta_ij|. */
+#define QSV_LA_DEV_HERMITIAN 0
+#define QSV_LA_DEV_OFFDIAG   1
+#define QSV_LA_DEV_IDENTITY  2
+int qsv_la_deviation(int kind, const qsv_mat* A, double* out);
+/* solve-linear-system / inverse (protocols.clj:284-304): Gauss-Jordan with
+ * partial pivoting on [A | B] (B = NULL: the identity, X = A^-1); a pivot of
+ * magnitude below 1e-14 fails with QSV_E_INVALID ("Matrix is singular"). */
+int qsv_la_solve(const qsv_mat* A, const qsv_mat* B, qsv_mat* X);
+/* eigen-hermitian (protocols.clj:363-373): eigenvalues (n doubles, in the
+ * order of V's columns; the caller sorts) and eigenvectors as the columns of
+ * V, by parallel cyclic Jacobi; sweeps (may be NULL) reports the sweeps used. */
+int qsv_la_eigh(const qsv_mat* A, double* eigenvalues, qsv_mat* V, int* sweeps);
+/* matrix-vector-product of a Kronecker product kept as its factors: x <-
+ * (F_0 kron ... kron F_{n-1}) x, each factor a row-major 2x2 (re, im) or NULL
+ * for the identity, factor k acting on reference qubit k. The shape the
+ * reference builds in expand-gate-to-n-qubits (src/org/soulspace/qclojure/domain/gate.clj:323-353),
+ * applied in O(2^n) per factor instead of through the 2^n x 2^n matrix. */
+int qsv_la_kron_apply(const double* const* factors, int nfactors, qsv_mat* x);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,227 @@
+"""Test infrastructure only: a CPU restatement (numpy, FP64) of the reference's
+fastmath complex linear-algebra backend
+(src/org/soulspace/qclojure/domain/math/fastmath/complex_linear_algebra.clj),
+the checker for the device backend in qclojure_amd/la.py.
+
+Pinned in tests/test_la_oracle.py against the expectations of the reference's
+math/backend_test.clj (identity products, ||[3, 4]|| = 5, the Pauli-Z
+spectrum, Hermitian/unitary detection) and against LAPACK through
+numpy.linalg (solve, inv, eigh) on random matrices, plus the reference's own
+quirks derived from the source (the pivot rule and 1e-14 threshold, the
+complex eigen-hermitian's doubled real embedding with `distinct`).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+class OracleLAError(ValueError):
+    pass
+
+
+def matrix_add(A, B):
+    """matrix-add (fastmath/complex_linear_algebra.clj:326-338)."""
+    return np.asarray(A) + np.asarray(B)
+
+
+def matrix_subtract(A, B):
+    """matrix-subtract (:340-352)."""
+    return np.asarray(A) - np.asarray(B)
+
+
+def matrix_scale(A, alpha):
+    """matrix-scale (:354-367)."""
+    return np.asarray(A) * complex(alpha)
+
+
+def matrix_negate(A):
+    """matrix-negate (:369-378)."""
+    return -np.asarray(A)
+
+
+def matrix_multiply(A, B):
+    """matrix-multiply (:380-406): sum_k A_ik B_kj, accumulated in k order."""
+    A, B = np.asarray(A, complex), np.asarray(B, complex)
+    if A.shape[1] != B.shape[0]:
+        raise OracleLAError("Matrix dimensions incompatible for multiplication")
+    C = np.zeros((A.shape[0], B.shape[1]), complex)
+    for k in range(A.shape[1]):
+        C += np.outer(A[:, k], B[k, :])
+    return C
+
+
+def matrix_vector_product(A, x):
+    """matrix-vector-product (:408-431)."""
+    return matrix_multiply(A, np.asarray(x, complex).reshape(-1, 1))[:, 0]
+
+
+def outer_product(x, y):
+    """outer-product (:433-447): x_i conj(y_j)."""
+    return np.outer(np.asarray(x, complex), np.conj(np.asarray(y, complex)))
+
+
+def hadamard_product(A, B):
+    """hadamard-product (:449-461)."""
+    return np.asarray(A, complex) * np.asarray(B, complex)
+
+
+def kronecker_product(A, B):
+    """kronecker-product (:463-484): block (i, j) is A_ij B."""
+    A, B = np.asarray(A, complex), np.asarray(B, complex)
+    p, q = A.shape
+    r, s = B.shape
+    C = np.zeros((p * r, q * s), complex)
+    for i in range(p):
+        for j in range(q):
+            C[i * r:(i + 1) * r, j * s:(j + 1) * s] = A[i, j] * B
+    return C
+
+
+def transpose(A):
+    """transpose (:486-500)."""
+    return np.asarray(A).T.copy()
+
+
+def conjugate_transpose(A):
+    """conjugate-transpose (:502-516)."""
+    return np.conj(np.asarray(A)).T.copy()
+
+
+def trace(A):
+    """trace (:518-534)."""
+    A = np.asarray(A, complex)
+    if A.shape[0] != A.shape[1]:
+        raise OracleLAError("Matrix must be square for trace")
+    return complex(sum(A[i, i] for i in range(A.shape[0])))
+
+
+def inner_product(x, y):
+    """inner-product (:536-555): sum conj(x_i) y_i."""
+    x, y = np.asarray(x, complex), np.asarray(y, complex)
+    if x.size != y.size:
+        raise OracleLAError("Vectors must have same length")
+    return complex(np.sum(np.conj(x) * y))
+
+
+def norm2(x):
+    """norm2 (:557-571)."""
+    x = np.asarray(x, complex)
+    return float(np.sqrt(np.sum((np.conj(x) * x).real)))
+
+
+def hermitian(A, eps):
+    """hermitian? (:573-590): every |A_ij - conj(A_ji)| < eps."""
+    A = np.asarray(A, complex)
+    k = min(A.shape)
+    return bool(np.all(np.abs(A[:k, :k] - np.conj(A[:k, :k]).T) < eps))
+
+
+def diagonal(A, eps):
+    """diagonal? (:592-612): every off-diagonal |A_ij| < eps."""
+    A = np.asarray(A, complex)
+    n = A.shape[0]
+    off = A[:n, :n] - np.diag(np.diag(A[:n, :n]))
+    return bool(np.all(np.abs(off) < eps))
+
+
+def unitary(U, eps):
+    """unitary? (:614-640): every |(U^H U - I)_ij| < eps."""
+    U = np.asarray(U, complex)
+    P = matrix_multiply(conjugate_transpose(U), U)
+    return bool(np.all(np.abs(P - np.eye(P.shape[0])) < eps))
+
+
+def _pivot_row(aug, i, n):
+    best = i
+    for r in range(i, n):
+        if abs(aug[r, i]) > abs(aug[best, i]):
+            best = r
+    return best
+
+
+def solve_linear(A, b):
+    """solve-linear (:174-249): forward elimination with partial pivoting
+    (first row of largest magnitude), back substitution; pivots below 1e-14
+    are singular."""
+    A = np.asarray(A, complex)
+    n, m = A.shape
+    if n != m:
+        raise OracleLAError("Matrix must be square for solving")
+    b = np.asarray(b, complex).reshape(-1)
+    if b.size != n:
+        raise OracleLAError("Incompatible dimensions")
+    aug = np.hstack([A, b.reshape(-1, 1)])
+    for i in range(n):
+        p = _pivot_row(aug, i, n)
+        if abs(aug[p, i]) < 1e-14:
+            raise OracleLAError("Matrix is singular")
+        if p != i:
+            aug[[i, p]] = aug[[p, i]]
+        for r in range(i + 1, n):
+            f = aug[r, i] / aug[i, i]
+            aug[r, :] = aug[r, :] - f * aug[i, :]
+            aug[r, i] = 0
+    x = np.zeros(n, complex)
+    for k in range(n - 1, -1, -1):
+        s = sum(aug[k, j] * x[j] for j in range(k + 1, n))
+        if abs(aug[k, k]) < 1e-14:
+            raise OracleLAError("Matrix is singular or nearly singular")
+        x[k] = (aug[k, n] - s) / aug[k, k]
+    return x
+
+
+def inverse(A):
+    """inverse (:251-324): Gauss-Jordan on [A | I] with the same pivot rule."""
+    A = np.asarray(A, complex)
+    n, m = A.shape
+    if n != m:
+        raise OracleLAError("Matrix must be square for inverse")
+    aug = np.hstack([A, np.eye(n, dtype=complex)])
+    for i in range(n):
+        p = _pivot_row(aug, i, n)
+        if abs(aug[p, i]) < 1e-14:
+            raise OracleLAError("Matrix is singular")
+        if p != i:
+            aug[[i, p]] = aug[[p, i]]
+        aug[i, :] = aug[i, :] / aug[i, i]
+        for r in range(n):
+            if r != i:
+                f = aug[r, i]
+                aug[r, :] = aug[r, :] - f * aug[i, :]
+                aug[r, i] = 0
+    return aug[:, n:].copy()
+
+
+def eigen_hermitian(A):
+    """eigen-hermitian (:642-756). Real input: the symmetric eigensystem,
+    ascending. Complex input: the real 2n x 2n embedding [[Re, -Im], [Im, Re]],
+    whose spectrum holds every eigenvalue twice, reduced with `distinct` (exact
+    equality) and one eigenvector (first n components + i * last n) per value."""
+    A = np.asarray(A, complex)
+    n = A.shape[0]
+    if A.shape[1] != n:
+        raise OracleLAError("Matrix must be square for eigendecomposition")
+    if np.all(A.imag == 0):
+        w, V = np.linalg.eigh(A.real)
+        order = np.argsort(w, kind="stable")
+        return {"eigenvalues": [complex(w[i]) for i in order], "eigenvectors": [V[:, i].astype(complex) for i in order]}
+    E = np.block([[A.real, -A.imag], [A.imag, A.real]])
+    w, V = np.linalg.eigh(E)
+    vals, reps = [], []
+    for idx, v in enumerate(w):
+        if v not in vals:
+            vals.append(v)
+            reps.append(idx)
+    vecs = [V[:n, i] + 1j * V[n:, i] for i in reps]
+    order = np.argsort(vals, kind="stable")
+    return {"eigenvalues": [complex(vals[i]) for i in order], "eigenvectors": [vecs[i] for i in order]}
+
+
+def positive_semidefinite(A, eps):
+    """positive-semidefinite? (:758-790)."""
+    A = np.asarray(A, complex)
+    if A.shape[0] != A.shape[1]:
+        raise OracleLAError("Error checking positive semidefinite property")
+    if not hermitian(A, eps):
+        raise OracleLAError("Error checking positive semidefinite property")
+    return all(v.real >= -eps for v in eigen_hermitian(A)["eigenvalues"])

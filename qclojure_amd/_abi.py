@@ -156,7 +156,30 @@ SIGNATURES = {
     "qsv_timing_read": (_i32, [_vp, _vp, _i32]),
     "qsv_tune_fused": (_i32, [_i32, _i32, _vp]),
     "qsv_noisy_trajectories": (_i32, [_i32, _i32, _vp, _u64, _vp, _u64, _u64, _u64, _vp, _u64, _vp]),
+    "qsv_mat_create": (_i32, [ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(_vp)]),
+    "qsv_mat_destroy": (_i32, [_vp]),
+    "qsv_mat_upload": (_i32, [_vp, _vp]),
+    "qsv_mat_download": (_i32, [_vp, _vp]),
+    "qsv_mat_copy": (_i32, [_vp, _vp]),
+    "qsv_mat_shape": (_i32, [_vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
+    "qsv_mat_view_state": (_i32, [_vp, ctypes.POINTER(_vp)]),
+    "qsv_la_axpby": (_i32, [_dbl, _dbl, _vp, _dbl, _dbl, _vp, _vp]),
+    "qsv_la_hadamard": (_i32, [_vp, _vp, _vp]),
+    "qsv_la_gemm": (_i32, [_i32, _vp, _vp, _vp]),
+    "qsv_la_gemv": (_i32, [_vp, _vp, _vp]),
+    "qsv_la_dot": (_i32, [_i32, _vp, _vp, _pd]),
+    "qsv_la_norm2": (_i32, [_vp, _pd]),
+    "qsv_la_outer": (_i32, [_vp, _vp, _vp]),
+    "qsv_la_kron": (_i32, [_vp, _vp, _vp]),
+    "qsv_la_transpose": (_i32, [_i32, _vp, _vp]),
+    "qsv_la_trace": (_i32, [_vp, _pd]),
+    "qsv_la_deviation": (_i32, [_i32, _vp, _pd]),
+    "qsv_la_solve": (_i32, [_vp, _vp, _vp]),
+    "qsv_la_eigh": (_i32, [_vp, _pd, _vp, ctypes.POINTER(_i32)]),
+    "qsv_la_kron_apply": (_i32, [ctypes.POINTER(_pd), _i32, _vp]),
 }
+
+QSV_LA_DEV_HERMITIAN, QSV_LA_DEV_OFFDIAG, QSV_LA_DEV_IDENTITY = 0, 1, 2
 
 
 class QsvError(RuntimeError):

@@ -844,6 +844,20 @@ constexpr uint64_t kXferChunk = 1ull << 24;  // amplitudes per staged transfer
 
 }  // namespace
 
+namespace qsv {
+int api_fail(int code, const char* msg) { return fail(code, msg); }
+int api_ensure_init() { return ensure_init(); }
+int api_state_raw(const ::qsv_state* s, void** amps, hipStream_t* st, int* n, int* prec)
+{
+  if (!s) return fail(QSV_E_INVALID, "null state");
+  *amps = s->amps;
+  *st = s->stream;
+  *n = s->n;
+  *prec = s->prec;
+  return QSV_OK;
+}
+}  // namespace qsv
+
 extern "C" {
 
 int qsv_init(int device)

@@ -6,6 +6,8 @@
 #include <vector>
 #include "../../include/qsv.h"
 
+struct qsv_state;
+
 namespace qsv {
 
 constexpr int kBlock = 256;          // threads per workgroup (4 waves of 64)
@@ -202,4 +204,11 @@ struct SampleJob {
 };
 void launch_batch_sample(int prec, const void* arena, const void* jobs, unsigned njobs, uint64_t* outcomes,
                          hipStream_t st);
+// Shared with the linear-algebra module (qsv_la.hip): set the calling
+// thread's error message and return `code`; make sure the runtime is
+// initialised; reach a state's amplitude buffer and stream.
+int api_fail(int code, const char* msg);
+int api_ensure_init();
+int api_state_raw(const struct ::qsv_state* s, void** amps, hipStream_t* st, int* n, int* prec);
+
 }  // namespace qsv

@@ -1,0 +1,1026 @@
+// qsv_la.hip — complex128 linear algebra on the device: the MatrixAlgebra /
+// eigen-hermitian contract of src/org/soulspace/qclojure/domain/math/protocols.clj:81-375
+// as kernels over row-major interleaved (re, im) matrices in HBM.
+//
+// * elementwise ops, transposes, Kronecker and outer products: one pass over
+//   the output, HBM-bound;
+// * matrix-multiply: FP64 MFMA (v_mfma_f64_16x16x4f64) on 64x64 output tiles
+//   staged through LDS as split real/imaginary planes, four real products per
+//   complex one;
+// * matrix-vector product: one wave per row, HBM-bound;
+// * reductions (inner product, norm, trace, predicate deviations): fixed grid,
+//   FP64 partials, fixed-order final sum, so results repeat run to run;
+// * solve / inverse: Gauss-Jordan with partial pivoting (the pivot rule and the
+//   1e-14 singularity threshold of fastmath/complex_linear_algebra.clj:174-324),
+//   four small launches per column, a device flag instead of a host sync;
+// * eigen-hermitian: parallel cyclic Jacobi (round-robin pairing, n/2 disjoint
+//   complex rotations per round), convergence checked once per sweep.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "qsv_internal.h"
+#include "../../include/qsv.h"
+
+// Products are rounded and then added, as the JVM computes them (no fused
+// multiply-add), so elementwise results equal the reference's bit for bit.
+#pragma clang fp contract(off)
+
+struct qsv_mat {
+  int64_t rows = 0, cols = 0;
+  double2* data = nullptr;
+  bool owns = true;
+  int device = 0;
+};
+
+namespace {
+
+using qsv::api_fail;
+
+constexpr int kLaBlock = 256;
+constexpr int kLaReduceBlocks = 1024;
+
+std::mutex g_la_mu;
+hipStream_t g_la_stream = nullptr;
+double* g_la_partials = nullptr;   // kLaReduceBlocks * 2
+double* g_la_out = nullptr;        // 8 doubles
+int* g_la_flag = nullptr;          // Gauss-Jordan singular flag + pivot index
+double2* g_la_piv = nullptr;       // Gauss-Jordan pivot value of the current column
+
+#define LACHK(x)                                                                              \
+  do {                                                                                        \
+    hipError_t e_ = (x);                                                                      \
+    if (e_ != hipSuccess) return api_fail(QSV_E_DEVICE, (std::string(#x) + ": " + hipGetErrorString(e_)).c_str()); \
+  } while (0)
+
+int la_setup()
+{
+  int rc = qsv::api_ensure_init();
+  if (rc) return rc;
+  if (g_la_stream) return QSV_OK;
+  LACHK(hipStreamCreateWithFlags(&g_la_stream, hipStreamNonBlocking));
+  LACHK(hipMalloc(&g_la_partials, sizeof(double) * kLaReduceBlocks * 2));
+  LACHK(hipMalloc(&g_la_out, sizeof(double) * 8));
+  LACHK(hipMalloc(&g_la_flag, sizeof(int) * 4));
+  LACHK(hipMalloc(&g_la_piv, sizeof(double2)));
+  return QSV_OK;
+}
+
+__device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ inline double cabs2(double2 a) { return a.x * a.x + a.y * a.y; }
+__device__ inline double cabs_(double2 a) { return sqrt(a.x * a.x + a.y * a.y); }
+// fastmath's complex division: (a * conj(b)) / |b|^2
+__device__ inline double2 cdiv(double2 a, double2 b)
+{
+  const double d = b.x * b.x + b.y * b.y;
+  return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+
+// ---- elementwise --------------------------------------------------------------------------
+__global__ void k_axpby(const double2* __restrict__ A, const double2* __restrict__ B, double2* __restrict__ C,
+                        uint64_t n, double2 al, double2 be, int hasB)
+{
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    double2 v = cmul(al, A[i]);
+    if (hasB) v = cadd(v, cmul(be, B[i]));
+    C[i] = v;
+  }
+}
+
+__global__ void k_hadamard(const double2* __restrict__ A, const double2* __restrict__ B, double2* __restrict__ C,
+                           uint64_t n)
+{
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    C[i] = cmul(A[i], B[i]);
+}
+
+// 32x32 tiles through LDS (one padding column against bank conflicts).
+__global__ void k_transpose(const double2* __restrict__ A, double2* __restrict__ C, int64_t rows, int64_t cols,
+                            int conj)
+{
+  __shared__ double2 tile[32][33];
+  const int64_t bx = (int64_t)blockIdx.x * 32, by = (int64_t)blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8 threads
+  for (int r = ty; r < 32; r += 8) {
+    const int64_t i = by + r, j = bx + tx;
+    if (i < rows && j < cols) tile[r][tx] = A[i * cols + j];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int64_t i = bx + r, j = by + tx;   // C is cols x rows
+    if (i < cols && j < rows) {
+      double2 v = tile[tx][r];
+      if (conj) v.y = -v.y;
+      C[i * rows + j] = v;
+    }
+  }
+}
+
+// C = A (p x q) kron B (r x s): C[i][j] = A[i / r][j / s] * B[i % r][j % s].
+__global__ void k_kron(const double2* __restrict__ A, const double2* __restrict__ B, double2* __restrict__ C,
+                       int64_t p, int64_t q, int64_t r, int64_t s)
+{
+  const int64_t cols = q * s, total = p * r * cols;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / cols, j = e - i * cols;
+    C[e] = cmul(A[(i / r) * q + j / s], B[(i % r) * s + j % s]);
+  }
+}
+
+// C[i][j] = x[i] * conj(y[j])   (fastmath/complex_linear_algebra.clj:433-447)
+__global__ void k_outer(const double2* __restrict__ x, const double2* __restrict__ y, double2* __restrict__ C,
+                        int64_t m, int64_t n)
+{
+  const int64_t total = m * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / n, j = e - i * n;
+    C[e] = cmul(x[i], cconj(y[j]));
+  }
+}
+
+// x <- (I..G_k..I) x for a 2x2 G on index bit b (a lazily kept Kronecker factor).
+__global__ void k_apply2(double2* __restrict__ x, uint64_t half, int b, double2 g00, double2 g01, double2 g10,
+                         double2 g11)
+{
+  const uint64_t lo = (1ull << b) - 1;
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < half; t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i0 = ((t & ~lo) << 1) | (t & lo), i1 = i0 | (1ull << b);
+    const double2 a0 = x[i0], a1 = x[i1];
+    x[i0] = cadd(cmul(g00, a0), cmul(g01, a1));
+    x[i1] = cadd(cmul(g10, a0), cmul(g11, a1));
+  }
+}
+
+// ---- GEMM on FP64 MFMA --------------------------------------------------------------------
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int GBM = 64, GBN = 64, GBK = 16;
+
+// C (M x N) = op(A) (M x K) * B (K x N); op = identity or conjugate transpose
+// (then A is stored K x M). Four waves per workgroup, each owning a 32 x 32
+// block of C as 2 x 2 MFMA tiles of 16 x 16 with separate real and imaginary
+// accumulators. MFMA operand layout (16x16x4, FP64): lane l supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16]; it receives C[4 r + l / 16][l % 16]
+// in accumulator element r (rows interleaved across the four lane groups).
+template <bool CONJA>
+__global__ __launch_bounds__(256) void k_zgemm(const double2* __restrict__ A, const double2* __restrict__ B,
+                                               double2* __restrict__ C, int M, int N, int K)
+{
+  __shared__ double sAr[GBK][GBM + 1], sAi[GBK][GBM + 1];
+  __shared__ double sBr[GBK][GBN + 1], sBi[GBK][GBN + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int i0 = blockIdx.y * GBM, j0 = blockIdx.x * GBN;
+  v4d accr[2][2], acci[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      accr[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+      acci[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+    }
+  for (int k0 = 0; k0 < K; k0 += GBK) {
+#pragma unroll
+    for (int e = tid; e < GBM * GBK; e += 256) {
+      int ii, kk;
+      if (CONJA) { kk = e / GBM; ii = e % GBM; }
+      else { ii = e / GBK; kk = e % GBK; }
+      const int gi = i0 + ii, gk = k0 + kk;
+      double2 v = make_double2(0.0, 0.0);
+      if (gi < M && gk < K) v = CONJA ? A[(size_t)gk * M + gi] : A[(size_t)gi * K + gk];
+      sAr[kk][ii] = v.x;
+      sAi[kk][ii] = CONJA ? -v.y : v.y;
+    }
+#pragma unroll
+    for (int e = tid; e < GBK * GBN; e += 256) {
+      const int kk = e / GBN, jj = e % GBN;
+      const int gk = k0 + kk, gj = j0 + jj;
+      double2 v = make_double2(0.0, 0.0);
+      if (gk < K && gj < N) v = B[(size_t)gk * N + gj];
+      sBr[kk][jj] = v.x;
+      sBi[kk][jj] = v.y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < GBK; ks += 4) {
+      const int kk = ks + (lane >> 4), r16 = lane & 15;
+      double ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const int ii = wm * 32 + a * 16 + r16;
+        ar[a] = sAr[kk][ii];
+        ai[a] = sAi[kk][ii];
+      }
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int jj = wn * 32 + b * 16 + r16;
+        br[b] = sBr[kk][jj];
+        bi[b] = sBi[kk][jj];
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], accr[a][b], 0, 0, 0);
+          accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[a], bi[b], accr[a][b], 0, 0, 0);
+          acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[b], acci[a][b], 0, 0, 0);
+          acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], br[b], acci[a][b], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gi = i0 + wm * 32 + a * 16 + 4 * r + (lane >> 4);
+        const int gj = j0 + wn * 32 + b * 16 + (lane & 15);
+        if (gi < M && gj < N) C[(size_t)gi * N + gj] = make_double2(accr[a][b][r], acci[a][b][r]);
+      }
+}
+
+// y (m) = A (m x n) x (n): one wave per row, lanes stride the row.
+__global__ void k_gemv(const double2* __restrict__ A, const double2* __restrict__ x, double2* __restrict__ y,
+                       int64_t m, int64_t n)
+{
+  const int lane = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (row >= m) return;
+  const double2* a = A + row * n;
+  double re = 0.0, im = 0.0;
+  for (int64_t j = lane; j < n; j += 64) {
+    const double2 u = a[j], v = x[j];
+    re += u.x * v.x - u.y * v.y;
+    im += u.x * v.y + u.y * v.x;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    re += __shfl_xor(re, off);
+    im += __shfl_xor(im, off);
+  }
+  if (lane == 0) y[row] = make_double2(re, im);
+}
+
+// ---- reductions ---------------------------------------------------------------------------
+enum RedKind { R_DOT = 0, R_DOTC = 1, R_NORM2 = 2, R_TRACE = 3, R_DEV_HERM = 4, R_DEV_OFFDIAG = 5, R_DEV_IDENT = 6,
+               R_OFFNORM = 7 };
+
+__device__ inline void block_reduce2(double& a, double& b, bool is_max)
+{
+  __shared__ double sa[kLaBlock / 64], sb[kLaBlock / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double oa = __shfl_xor(a, off), ob = __shfl_xor(b, off);
+    a = is_max ? fmax(a, oa) : a + oa;
+    b = is_max ? fmax(b, ob) : b + ob;
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) { sa[w] = a; sb[w] = b; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < kLaBlock / 64; ++i) {
+      a = is_max ? fmax(a, sa[i]) : a + sa[i];
+      b = is_max ? fmax(b, sb[i]) : b + sb[i];
+    }
+  }
+}
+
+// Partial (sum or max) over the elements of one matrix or a pair of vectors.
+// x/y: operands; rows/cols: shape of x (vectors: count x 1).
+__global__ void k_reduce(int kind, const double2* __restrict__ x, const double2* __restrict__ y, int64_t rows,
+                         int64_t cols, double* __restrict__ partials)
+{
+  const bool is_max = kind >= R_DEV_HERM && kind <= R_DEV_IDENT;
+  double a = 0.0, b = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (kind == R_TRACE) {
+    const int64_t k = rows < cols ? rows : cols;
+    for (int64_t i = t0; i < k; i += stride) {
+      const double2 v = x[i * cols + i];
+      a += v.x;
+      b += v.y;
+    }
+  } else if (kind <= R_NORM2) {
+    const int64_t n = rows * cols;
+    for (int64_t i = t0; i < n; i += stride) {
+      const double2 u = x[i];
+      if (kind == R_NORM2) {
+        a += u.x * u.x + u.y * u.y;
+      } else {
+        const double2 v = y[i];
+        const double ur = u.x, ui = kind == R_DOTC ? -u.y : u.y;
+        a += ur * v.x - ui * v.y;
+        b += ur * v.y + ui * v.x;
+      }
+    }
+  } else {
+    // deviations over the leading k x k block (k = rows, cols >= rows checked on the host)
+    const int64_t k = kind == R_DEV_IDENT || kind == R_OFFNORM ? rows : (rows < cols ? rows : cols);
+    const int64_t n = k * k;
+    for (int64_t e = t0; e < n; e += stride) {
+      const int64_t i = e / k, j = e - i * k;
+      const double2 v = x[i * cols + j];
+      if (kind == R_DEV_HERM) {
+        const double2 w = x[j * cols + i];
+        a = fmax(a, cabs_(make_double2(v.x - w.x, v.y + w.y)));
+      } else if (kind == R_DEV_OFFDIAG) {
+        if (i != j) a = fmax(a, cabs_(v));
+      } else if (kind == R_DEV_IDENT) {
+        a = fmax(a, cabs_(make_double2(v.x - (i == j ? 1.0 : 0.0), v.y)));
+      } else {   // R_OFFNORM: off-diagonal and total squared Frobenius mass
+        const double m2 = cabs2(v);
+        if (i != j) a += m2;
+        b += m2;
+      }
+    }
+  }
+  block_reduce2(a, b, is_max);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = a;
+    partials[2 * blockIdx.x + 1] = b;
+  }
+}
+
+__global__ void k_reduce_final(const double* __restrict__ partials, int nblocks, int is_max, double* __restrict__ out)
+{
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < nblocks; i += blockDim.x) {
+    a = is_max ? fmax(a, partials[2 * i]) : a + partials[2 * i];
+    b = is_max ? fmax(b, partials[2 * i + 1]) : b + partials[2 * i + 1];
+  }
+  block_reduce2(a, b, is_max != 0);
+  if (threadIdx.x == 0) {
+    out[0] = a;
+    out[1] = b;
+  }
+}
+
+int reduce(int kind, const double2* x, const double2* y, int64_t rows, int64_t cols, double out[2])
+{
+  const bool is_max = kind >= R_DEV_HERM && kind <= R_DEV_IDENT;
+  hipLaunchKernelGGL(k_reduce, dim3(kLaReduceBlocks), dim3(kLaBlock), 0, g_la_stream, kind, x, y, rows, cols,
+                     g_la_partials);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kLaBlock), 0, g_la_stream, (const double*)g_la_partials,
+                     kLaReduceBlocks, is_max ? 1 : 0, g_la_out);
+  LACHK(hipGetLastError());
+  LACHK(hipMemcpyAsync(out, g_la_out, 2 * sizeof(double), hipMemcpyDeviceToHost, g_la_stream));
+  LACHK(hipStreamSynchronize(g_la_stream));
+  return QSV_OK;
+}
+
+// ---- Gauss-Jordan with partial pivoting on M = [A | B] (n x w) ------------------------------
+// flag[0]: 0, or 1 + the first column whose pivot magnitude fell below 1e-14;
+// flag[1]: the pivot row of the current column; piv: its value, saved before
+// the swap kernel overwrites the column.
+__global__ void k_gj_pivot(const double2* __restrict__ M, int64_t n, int64_t w, int64_t col, int* __restrict__ flag,
+                           double2* __restrict__ piv)
+{
+  __shared__ double sv[kLaBlock];
+  __shared__ int64_t si[kLaBlock];
+  if (flag[0]) return;
+  double best = -1.0;
+  int64_t bi = col;
+  for (int64_t r = col + threadIdx.x; r < n; r += blockDim.x) {
+    const double v = cabs_(M[r * w + col]);
+    if (v > best) { best = v; bi = r; }   // rows visited in increasing order per thread
+  }
+  sv[threadIdx.x] = best;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      const double ov = sv[threadIdx.x + s];
+      const int64_t oi = si[threadIdx.x + s];
+      // larger magnitude wins; equal magnitudes keep the lower row (the first maximum)
+      if (ov > sv[threadIdx.x] || (ov == sv[threadIdx.x] && oi < si[threadIdx.x])) {
+        sv[threadIdx.x] = ov;
+        si[threadIdx.x] = oi;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (sv[0] < 1e-14) flag[0] = (int)(col + 1);
+    flag[1] = (int)si[0];
+    *piv = M[si[0] * w + col];
+  }
+}
+
+// Swap rows col and p, then divide the pivot row by the pivot. Each thread
+// owns whole columns c of both rows.
+__global__ void k_gj_swap_scale(double2* __restrict__ M, int64_t w, int64_t col, const int* __restrict__ flag,
+                                const double2* __restrict__ pivp)
+{
+  if (flag[0]) return;
+  const int64_t p = flag[1];
+  const double2 piv = *pivp;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < w; c += (int64_t)gridDim.x * blockDim.x) {
+    const double2 top = M[col * w + c], other = M[p * w + c];
+    if (p != col) M[p * w + c] = top;
+    M[col * w + c] = cdiv(other, piv);
+  }
+}
+
+__global__ void k_gj_factors(const double2* __restrict__ M, int64_t n, int64_t w, int64_t col,
+                             double2* __restrict__ f, const int* __restrict__ flag)
+{
+  if (flag[0]) return;
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+    f[r] = r == col ? make_double2(0.0, 0.0) : M[r * w + col];
+}
+
+__global__ void k_gj_eliminate(double2* __restrict__ M, int64_t n, int64_t w, int64_t col,
+                               const double2* __restrict__ f, const int* __restrict__ flag)
+{
+  if (flag[0]) return;
+  const int64_t total = n * w;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = e / w, c = e - r * w;
+    if (r == col) continue;
+    if (c == col) M[e] = make_double2(0.0, 0.0);
+    else M[e] = csub(M[e], cmul(f[r], M[col * w + c]));
+  }
+}
+
+__global__ void k_gj_build(const double2* __restrict__ A, const double2* __restrict__ B, double2* __restrict__ M,
+                           int64_t n, int64_t k, int identity)
+{
+  const int64_t w = n + k, total = n * w;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = e / w, c = e - r * w;
+    if (c < n) M[e] = A[r * n + c];
+    else if (identity) M[e] = make_double2(r == c - n ? 1.0 : 0.0, 0.0);
+    else M[e] = B[r * k + (c - n)];
+  }
+}
+
+__global__ void k_gj_extract(const double2* __restrict__ M, double2* __restrict__ X, int64_t n, int64_t k)
+{
+  const int64_t w = n + k, total = n * k;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = e / k, c = e - r * k;
+    X[e] = M[r * w + n + c];
+  }
+}
+
+// ---- parallel cyclic Jacobi for Hermitian matrices -------------------------------------------
+// Rotation for pair (p, q) with b = M[p][q] = |b| e^{i phi}: U = D J with
+// D = diag(1, e^{-i phi}) (makes the pair block real) and the symmetric Schur
+// rotation J = [[c, s], [-s, c]] (tan of the angle t = sgn(tau)/(|tau| + sqrt(1 + tau^2)),
+// tau = (M_qq - M_pp) / (2 |b|)). A <- U^H A U, V <- V U.
+struct Rot {
+  double c, s, er, ei;   // E = e^{-i phi}
+};
+
+__global__ void k_jac_params(const double2* __restrict__ M, int64_t n, const int2* __restrict__ pairs, int npairs,
+                             Rot* __restrict__ rot)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= npairs) return;
+  const int p = pairs[k].x, q = pairs[k].y;
+  Rot R{1.0, 0.0, 1.0, 0.0};
+  if (p < n && q < n) {
+    // the Hermitian part of the pair's off-diagonal entry (rounding leaves
+    // M_qp and conj(M_pq) a few ulps apart)
+    const double2 bu = M[(int64_t)p * n + q], bl = M[(int64_t)q * n + p];
+    const double2 b = make_double2(0.5 * (bu.x + bl.x), 0.5 * (bu.y - bl.y));
+    // |b| and the phase e^{-i phi} with scaling, so that tiny entries (whose
+    // squares would be subnormal) still give a phase of modulus exactly 1 up
+    // to rounding; entries below 1e-290 are treated as zero.
+    const double sc = fmax(fabs(b.x), fabs(b.y));
+    if (sc > 1e-290) {
+      const double bx = b.x / sc, by = b.y / sc;
+      const double r = sqrt(bx * bx + by * by);
+      const double ab = sc * r;
+      const double app = M[(int64_t)p * n + p].x, aqq = M[(int64_t)q * n + q].x;
+      const double tau = (aqq - app) / (2.0 * ab);
+      const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+      const double c = 1.0 / sqrt(1.0 + t * t);
+      R = Rot{c, t * c, bx / r, -by / r};
+    }
+  }
+  rot[k] = R;
+}
+
+// Rows: A_p <- c A_p - s conj(E) A_q ; A_q <- s A_p + c conj(E) A_q.
+__global__ void k_jac_rows(double2* __restrict__ M, int64_t n, const int2* __restrict__ pairs, int npairs,
+                           const Rot* __restrict__ rot)
+{
+  const int64_t total = (int64_t)npairs * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int k = (int)(e / n);
+    const int64_t j = e - (int64_t)k * n;
+    const int p = pairs[k].x, q = pairs[k].y;
+    if (p >= n || q >= n) continue;
+    const Rot R = rot[k];
+    const double2 Ec = make_double2(R.er, -R.ei);
+    const double2 ap = M[(int64_t)p * n + j], aq = M[(int64_t)q * n + j];
+    const double2 eq = cmul(Ec, aq);
+    M[(int64_t)p * n + j] = make_double2(R.c * ap.x - R.s * eq.x, R.c * ap.y - R.s * eq.y);
+    M[(int64_t)q * n + j] = make_double2(R.s * ap.x + R.c * eq.x, R.s * ap.y + R.c * eq.y);
+  }
+}
+
+// Columns of A and V: X_:p <- c X_:p - s E X_:q ; X_:q <- s X_:p + c E X_:q.
+// The pair's off-diagonal entries, which the rotation annihilates, are stored
+// as exact zeros and its diagonal entries as real numbers (thread (p, k) owns
+// row p of the pair's two columns, thread (q, k) row q).
+__global__ void k_jac_cols(double2* __restrict__ M, double2* __restrict__ V, int64_t n,
+                           const int2* __restrict__ pairs, int npairs, const Rot* __restrict__ rot)
+{
+  const int64_t total = (int64_t)npairs * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / npairs;
+    const int k = (int)(e - i * npairs);
+    const int p = pairs[k].x, q = pairs[k].y;
+    if (p >= n || q >= n) continue;
+    const Rot R = rot[k];
+    const double2 E = make_double2(R.er, R.ei);
+    double2* X[2] = {M, V};
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      const double2 xp = X[m][i * n + p], xq = X[m][i * n + q];
+      const double2 eq = cmul(E, xq);
+      double2 np_ = make_double2(R.c * xp.x - R.s * eq.x, R.c * xp.y - R.s * eq.y);
+      double2 nq_ = make_double2(R.s * xp.x + R.c * eq.x, R.s * xp.y + R.c * eq.y);
+      if (m == 0 && i == p) { np_.y = 0.0; nq_ = make_double2(0.0, 0.0); }
+      if (m == 0 && i == q) { np_ = make_double2(0.0, 0.0); nq_.y = 0.0; }
+      X[m][i * n + p] = np_;
+      X[m][i * n + q] = nq_;
+    }
+  }
+}
+
+__global__ void k_identity(double2* __restrict__ V, int64_t n)
+{
+  const int64_t total = n * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / n, j = e - i * n;
+    V[e] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+  }
+}
+
+__global__ void k_diag_real(const double2* __restrict__ M, int64_t n, double* __restrict__ w)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    w[i] = M[i * n + i].x;
+}
+
+unsigned grid_for(uint64_t n)
+{
+  const uint64_t b = (n + kLaBlock - 1) / kLaBlock;
+  return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(b, 65536));
+}
+
+int check_mat(const qsv_mat* m, const char* what)
+{
+  if (!m || !m->data) return api_fail(QSV_E_INVALID, (std::string("null matrix: ") + what).c_str());
+  return QSV_OK;
+}
+
+int alloc(double2** p, uint64_t count)
+{
+  if (hipMalloc(p, std::max<uint64_t>(count, 1) * sizeof(double2)) != hipSuccess)
+    return api_fail(QSV_E_NOMEM, ("hipMalloc of " + std::to_string(count * 16) + " B failed").c_str());
+  return QSV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qsv_mat_create(int64_t rows, int64_t cols, qsv_mat** out)
+{
+  if (!out) return api_fail(QSV_E_INVALID, "null output");
+  *out = nullptr;
+  if (rows < 1 || cols < 1) return api_fail(QSV_E_INVALID, "matrix dimensions must be positive");
+  std::lock_guard<std::mutex> lk(g_la_mu);
+  int rc = la_setup();
+  if (rc) return rc;
+  auto* m = new qsv_mat();
+  m->rows = rows;
+  m->cols = cols;
+  if ((rc = alloc(&m->data, (uint64_t)rows * cols))) {
+    delete m;
+    return rc;
+  }
+  if (hipMemsetAsync(m->data, 0, (size_t)rows * cols * sizeof(double2), g_la_stream) != hipSuccess ||
+      hipStreamSynchronize(g_la_stream) != hipSuccess) {
+    hipFree(m->data);
+    delete m;
+    return api_fail(QSV_E_DEVICE, "clearing a new matrix failed");
+  }
+  *out = m;
+  return QSV_OK;
+}
+
+int qsv_mat_destroy(qsv_mat* m)
+{
+  if (!m) return QSV_OK;
+  if (m->owns && m->data) hipFree(m->data);
+  delete m;
+  return QSV_OK;
+}
+
+int qsv_mat_shape(const qsv_mat* m, int64_t* rows, int64_t* cols)
+{
+  if (!m || !rows || !cols) return api_fail(QSV_E_INVALID, "null argument");
+  *rows = m->rows;
+  *cols = m->cols;
+  return QSV_OK;
+}
+
+int qsv_mat_upload(qsv_mat* m, const double* host)
+{
+  int rc = check_mat(m, "upload");
+  if (rc) return rc;
+  if (!host) return api_fail(QSV_E_INVALID, "null host buffer");
+  std::lock_guard<std::mutex> lk(g_la_mu);
+  LACHK(hipMemcpyAsync(m->data, host, (size_t)m->rows * m->cols * sizeof(double2), hipMemcpyHostToDevice,
+                       g_la_stream));
+  LACHK(hipStreamSynchronize(g_la_stream));
+  return QSV_OK;
+}
+
+int qsv_mat_download(const qsv_mat* m, double* host)
+{
+  int rc = check_mat(m, "download");
+  if (rc) return rc;
+  if (!host) return api_fail(QSV_E_INVALID, "null host buffer");
+  std::lock_guard<std::mutex> lk(g_la_mu);
+  LACHK(hipMemcpyAsync(host, m->data, (size_t)m->rows * m->cols * sizeof(double2), hipMemcpyDeviceToHost,
+                       g_la_stream));
+  LACHK(hipStreamSynchronize(g_la_stream));
+  return QSV_OK;
+}
+
+int qsv_mat_copy(const qsv_mat* src, qsv_mat* dst)
+{
+  int rc = check_mat(src, "copy source");
+  if (rc || (rc = check_mat(dst, "copy destination"))) return rc;
+  if (src->rows * src->cols != dst->rows * dst->cols) return api_fail(QSV_E_INVALID, "copy: element counts differ");
+  std::lock_guard<std::mutex> lk(g_la_mu);
+  LACHK(hipMemcpyAsync(dst->data, src->data, (size_t)src->rows * src->cols * sizeof(double2),
+                       hipMemcpyDeviceToDevice, g_la_stream));
+  LACHK(hipStreamSynchronize(g_la_stream));
+  return QSV_OK;
+}
+
+int qsv_mat_view_state(qsv_state* s, qsv_mat** out)
+{
+  if (!out) return api_fail(QSV_E_INVALID, "null output");
+  void* amps = nullptr;
+  hipStream_t st = nullptr;
+  int n = 0, prec = 0;
+  int rc = qsv::api_state_raw(s, &amps, &st, &n, &prec);
+  if (rc) return rc;
+  if (prec != QSV_C128) return api_fail(QSV_E_UNSUPPORTED, "matrix views need a complex128 state");
+  std::lock_guard<std::mutex> lk(g_la_mu);
+  if ((rc = la_setup())) return rc;
+  LACHK(hipStreamSynchronize(st));
+  auto* m = new qsv_mat();
+  m->rows = (int64_t)1 << n;
+  m->cols = 1;
+  m->data = (double2*)amps;
+  m->owns = false;
+  *out = m;
+  return QSV_OK;
+}
+
+int qsv_la_axpby(double ar, double ai, const qsv_mat* A, double br, double bi, const qsv_mat* B, qsv_mat* C)
+{
+  int rc = check_mat(A, "A");
+  if (rc || (rc = check_mat(C, "C"))) return rc;
+  if (B && (B->rows != A->rows || B->cols != A->cols))
+    return api_fail(QSV_E_INVALID, "axpby: A and B shapes differ");
+  if (C->rows != A->rows || C->cols != A->cols) return api_fail(QSV_E_INVALID, "axpby: C shape differs");
+  std::lock_guard<std::mutex> lk(g_la_mu);
+  const uint64_t n = (uint64_t)A->rows * A->cols;
+  hipLaunchKernelGGL(k_axpby, dim3(grid_for(n)), dim3(kLaBlock), 0, g_la_stream, (const double2*)A->data,
+                     (const double2*)(B ? B->data : A->data), C->data, n, make_double2(ar, ai), make_double2(br, bi),
+                     B ? 1 : 0);
+  LACHK(hipGetLastError());
+  LACHK(hipStreamSynchronize(g_la_stream));
+  return QSV_OK;
+}
+
+int qsv_la_hadamard(const qsv_mat* A, const qsv_mat* B, qsv_mat* C)
+{
+  int rc = check_mat(A, "A");
+  if (rc || (rc = check_mat(B, "B")) || (rc = check_mat(C, "C"))) return rc;
+  if (B->rows != A->rows || B->cols != A->cols || C->rows != A->rows || C->cols != A->cols)
+    return api_fail(QSV_E_INVALID, "hadamard: shapes differ");
+  std::lock_guard<std::mutex> lk(g_la_mu);
+  const uint64_t n = (uint64_t)A->rows * A->cols;
+  hipLaunchKernelGGL(k_hadamard, dim3(grid_for(n)), dim3(kLaBlock), 0, g_la_stream, (const double2*)A->data,
+                     (const double2*)B->data, C->data, n);
+  LACHK(hipGetLastError());
+  LACHK(hipStreamSynchronize(g_la_stream));
+  return QSV_OK;
+}
+
+int qsv_la_gemm(int conj_transpose_a, const qsv_mat* A, const qsv_mat* B, qsv_mat* C)
+{
+  int rc = check_mat(A, "A");
+  if (rc || (rc = check_mat(B, "B")) || (rc = check_mat(C, "C"))) return rc;
+  const int64_t M = conj_transpose_a ? A->cols : A->rows, K = conj_transpose_a ? A->rows : A->cols;
+  if (B->rows != K) return api_fail(QSV_E_INVALID, "matrix-multiply: inner dimensions differ");
+  if (C->rows != M || C->cols != B->cols) return api_fail(QSV_E_INVALID, "matrix-multiply: C shape differs");
+  if (M > (1 << 30) || K > (1 << 30) || B->cols > (1 << 30) || (M + GBM - 1) / GBM > 65535)
+    return api_fail(QSV_E_INVALID, "matrix-multiply: dimension too large");
+  if (C->data == A->data || C->data == B->data) return api_fail(QSV_E_INVALID, "matrix-multiply: C aliases A or B");
+  std::lock_guard<std::mutex> lk(g_la_mu);
+  const dim3 grid((unsigned)((B->cols + GBN - 1) / GBN), (unsigned)((M + GBM - 1) / GBM));
+  if (conj_transpose_a)
+    hipLaunchKernelGGL(k_zgemm<true>, grid, dim3(256), 0, g_la_stream, (const double2*)A->data,
+                       (const double2*)B->data, C->data, (int)M, (int)B->cols, (int)K);
+  else
+    hipLaunchKernelGGL(k_zgemm<false>, grid, dim3(256), 0, g_la_stream, (const double2*)A->data,
+                       (const double2*)B->data, C->data, (int)M, (int)B->cols, (int)K);
+  LACHK(hipGetLastError());
+  LACHK(hipStreamSynchronize(g_la_stream));
+  return QSV_OK;
+}
+
+int qsv_la_gemv(const qsv_mat* A, const qsv_mat* x, qsv_mat* y)
+{
+  int rc = check_mat(A, "A");
+  if (rc || (rc = check_mat(x, "x")) || (rc = check_mat(y, "y"))) return rc;
+  if (x->rows * x->cols != A->cols) return api_fail(QSV_E_INVALID, "matrix-vector-product: length differs");
+  if (y->rows * y->cols != A->rows) return api_fail(QSV_E_INVALID, "matrix-vector-product: output length differs");
+  if (y->data == x->data) return api_fail(QSV_E_INVALID, "matrix-vector-product: y aliases x");
+  std::lock_guard<std::mutex> lk(g_la_mu);
+  const unsigned rows_per_block = kLaBlock / 64;
+  const unsigned grid = (unsigned)((A->rows + rows_per_block - 1) / rows_per_block);
+  hipLaunchKernelGGL(k_gemv, dim3(grid), dim3(kLaBlock), 0, g_la_stream, (const double2*)A->data,
+                     (const double2*)x->data, y->data, A->rows, A->cols);
+  LACHK(hipGetLastError());
+  LACHK(hipStreamSynchronize(g_la_stream));
+  return QSV_OK;
+}
+
+int qsv_la_dot(int conj_x, const qsv_mat* x, const qsv_mat* y, double out[2])
+{
+  int rc = check_mat(x, "x");
+  if (rc || (rc = check_mat(y, "y"))) return rc;
+  if (!out) return api_fail(QSV_E_INVALID, "null output");
+  if (x->rows * x->cols != y->rows * y->cols) return api_fail(QSV_E_INVALID, "inner-product: lengths differ");
+  std::lock_guard<std::mutex> lk(g_la_mu);
+  return reduce(conj_x ? R_DOTC : R_DOT, x->data, y->data, x->rows * x->cols, 1, out);
+}
+
+int qsv_la_norm2(const qsv_mat* x, double* out)
+{
+  int rc = check_mat(x, "x");
+  if (rc) return rc;
+  if (!out) return api_fail(QSV_E_INVALID, "null output");
+  std::lock_guard<std::mutex> lk(g_la_mu);
+  double r[2];
+  if ((rc = reduce(R_NORM2, x->data, nullptr, x->rows * x->cols, 1, r))) return rc;
+  *out = std::sqrt(r[0]);
+  return QSV_OK;
+}
+
+int qsv_la_outer(const qsv_mat* x, const qsv_mat* y, qsv_mat* C)
+{
+  int rc = check_mat(x, "x");
+  if (rc || (rc = check_mat(y, "y")) || (rc = check_mat(C, "C"))) return rc;
+  const int64_t m = x->rows * x->cols, n = y->rows * y->cols;
+  if (C->rows != m || C->cols != n) return api_fail(QSV_E_INVALID, "outer-product: C shape differs");
+  std::lock_guard<std::mutex> lk(g_la_mu);
+  hipLaunchKernelGGL(k_outer, dim3(grid_for((uint64_t)m * n)), dim3(kLaBlock), 0, g_la_stream,
+                     (const double2*)x->data, (const double2*)y->data, C->data, m, n);
+  LACHK(hipGetLastError());
+  LACHK(hipStreamSynchronize(g_la_stream));
+  return QSV_OK;
+}
+
+int qsv_la_kron(const qsv_mat* A, const qsv_mat* B, qsv_mat* C)
+{
+  int rc = check_mat(A, "A");
+  if (rc || (rc = check_mat(B, "B")) || (rc = check_mat(C, "C"))) return rc;
+  if (C->rows != A->rows * B->rows || C->cols != A->cols * B->cols)
+    return api_fail(QSV_E_INVALID, "kronecker-product: C shape differs");
+  std::lock_guard<std::mutex> lk(g_la_mu);
+  hipLaunchKernelGGL(k_kron, dim3(grid_for((uint64_t)C->rows * C->cols)), dim3(kLaBlock), 0, g_la_stream,
+                     (const double2*)A->data, (const double2*)B->data, C->data, A->rows, A->cols, B->rows, B->cols);
+  LACHK(hipGetLastError());
+  LACHK(hipStreamSynchronize(g_la_stream));
+  return QSV_OK;
+}
+
+int qsv_la_transpose(int conj, const qsv_mat* A, qsv_mat* C)
+{
+  int rc = check_mat(A, "A");
+  if (rc || (rc = check_mat(C, "C"))) return rc;
+  if (C->rows != A->cols || C->cols != A->rows) return api_fail(QSV_E_INVALID, "transpose: C shape differs");
+  if (C->data == A->data) return api_fail(QSV_E_INVALID, "transpose: C aliases A");
+  if ((A->cols + 31) / 32 > 0x7fffffff || (A->rows + 31) / 32 > 65535)
+    return api_fail(QSV_E_INVALID, "transpose: too many rows");
+  std::lock_guard<std::mutex> lk(g_la_mu);
+  const dim3 grid((unsigned)((A->cols + 31) / 32), (unsigned)((A->rows + 31) / 32));
+  hipLaunchKernelGGL(k_transpose, grid, dim3(256), 0, g_la_stream, (const double2*)A->data, C->data, A->rows,
+                     A->cols, conj);
+  LACHK(hipGetLastError());
+  LACHK(hipStreamSynchronize(g_la_stream));
+  return QSV_OK;
+}
+
+int qsv_la_trace(const qsv_mat* A, double out[2])
+{
+  int rc = check_mat(A, "A");
+  if (rc) return rc;
+  if (!out) return api_fail(QSV_E_INVALID, "null output");
+  if (A->rows != A->cols) return api_fail(QSV_E_INVALID, "Matrix must be square for trace");
+  std::lock_guard<std::mutex> lk(g_la_mu);
+  return reduce(R_TRACE, A->data, nullptr, A->rows, A->cols, out);
+}
+
+int qsv_la_deviation(int kind, const qsv_mat* A, double* out)
+{
+  int rc = check_mat(A, "A");
+  if (rc) return rc;
+  if (!out) return api_fail(QSV_E_INVALID, "null output");
+  int rk;
+  switch (kind) {
+    case QSV_LA_DEV_HERMITIAN: rk = R_DEV_HERM; break;
+    case QSV_LA_DEV_OFFDIAG:
+      rk = R_DEV_OFFDIAG;
+      if (A->cols < A->rows) return api_fail(QSV_E_INVALID, "diagonal?: fewer columns than rows");
+      break;
+    case QSV_LA_DEV_IDENTITY:
+      rk = R_DEV_IDENT;
+      if (A->cols != A->rows) return api_fail(QSV_E_INVALID, "identity deviation needs a square matrix");
+      break;
+    default: return api_fail(QSV_E_INVALID, "unknown deviation kind");
+  }
+  std::lock_guard<std::mutex> lk(g_la_mu);
+  double r[2];
+  if ((rc = reduce(rk, A->data, nullptr, A->rows, A->cols, r))) return rc;
+  *out = r[0];
+  return QSV_OK;
+}
+
+int qsv_la_solve(const qsv_mat* A, const qsv_mat* B, qsv_mat* X)
+{
+  int rc = check_mat(A, "A");
+  if (rc || (rc = check_mat(X, "X"))) return rc;
+  const int64_t n = A->rows;
+  if (A->cols != n) return api_fail(QSV_E_INVALID, B ? "Matrix must be square for solving"
+                                                     : "Matrix must be square for inverse");
+  const int64_t k = B ? B->cols : n;
+  if (B && B->rows != n) return api_fail(QSV_E_INVALID, "Incompatible dimensions");
+  if (X->rows != n || X->cols != k) return api_fail(QSV_E_INVALID, "solve: X shape differs");
+  std::lock_guard<std::mutex> lk(g_la_mu);
+  const int64_t w = n + k;
+  double2* M = nullptr;
+  double2* f = nullptr;
+  if ((rc = alloc(&M, (uint64_t)n * w))) return rc;
+  if ((rc = alloc(&f, (uint64_t)n))) {
+    hipFree(M);
+    return rc;
+  }
+  auto done = [&](int code) {
+    hipFree(M);
+    hipFree(f);
+    return code;
+  };
+  if (hipMemsetAsync(g_la_flag, 0, 4 * sizeof(int), g_la_stream) != hipSuccess)
+    return done(api_fail(QSV_E_DEVICE, "clearing the pivot flag failed"));
+  hipLaunchKernelGGL(k_gj_build, dim3(grid_for((uint64_t)n * w)), dim3(kLaBlock), 0, g_la_stream,
+                     (const double2*)A->data, (const double2*)(B ? B->data : A->data), M, n, k, B ? 0 : 1);
+  for (int64_t col = 0; col < n; ++col) {
+    hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(kLaBlock), 0, g_la_stream, (const double2*)M, n, w, col, g_la_flag,
+                       g_la_piv);
+    hipLaunchKernelGGL(k_gj_swap_scale, dim3(grid_for((uint64_t)w)), dim3(kLaBlock), 0, g_la_stream, M, w, col,
+                       (const int*)g_la_flag, (const double2*)g_la_piv);
+    hipLaunchKernelGGL(k_gj_factors, dim3(grid_for((uint64_t)n)), dim3(kLaBlock), 0, g_la_stream,
+                       (const double2*)M, n, w, col, f, (const int*)g_la_flag);
+    hipLaunchKernelGGL(k_gj_eliminate, dim3(grid_for((uint64_t)n * w)), dim3(kLaBlock), 0, g_la_stream, M, n, w,
+                       col, (const double2*)f, (const int*)g_la_flag);
+  }
+  hipLaunchKernelGGL(k_gj_extract, dim3(grid_for((uint64_t)n * k)), dim3(kLaBlock), 0, g_la_stream,
+                     (const double2*)M, X->data, n, k);
+  if (hipGetLastError() != hipSuccess) return done(api_fail(QSV_E_DEVICE, "Gauss-Jordan launch failed"));
+  int flag[2] = {0, 0};
+  if (hipMemcpyAsync(flag, g_la_flag, 2 * sizeof(int), hipMemcpyDeviceToHost, g_la_stream) != hipSuccess ||
+      hipStreamSynchronize(g_la_stream) != hipSuccess)
+    return done(api_fail(QSV_E_DEVICE, "Gauss-Jordan failed"));
+  if (flag[0]) {
+    const std::string msg = "Matrix is singular (pivot magnitude below 1e-14 in column " +
+                            std::to_string(flag[0] - 1) + ")";
+    return done(api_fail(QSV_E_INVALID, msg.c_str()));
+  }
+  return done(QSV_OK);
+}
+
+int qsv_la_eigh(const qsv_mat* A, double* eigenvalues, qsv_mat* V, int* sweeps_out)
+{
+  int rc = check_mat(A, "A");
+  if (rc || (rc = check_mat(V, "V"))) return rc;
+  if (!eigenvalues) return api_fail(QSV_E_INVALID, "null eigenvalue buffer");
+  const int64_t n = A->rows;
+  if (A->cols != n) return api_fail(QSV_E_INVALID, "Matrix must be square for eigendecomposition");
+  if (V->rows != n || V->cols != n) return api_fail(QSV_E_INVALID, "eigen-hermitian: V shape differs");
+  if (n > 16384) return api_fail(QSV_E_INVALID, "eigen-hermitian: matrix too large");
+  if (V->data == A->data) return api_fail(QSV_E_INVALID, "eigen-hermitian: V aliases A");
+  std::lock_guard<std::mutex> lk(g_la_mu);
+  // round-robin schedule: N (even) players, N - 1 rounds of N / 2 disjoint pairs
+  const int64_t N = n + (n & 1);
+  const int npairs = (int)(N / 2), nrounds = (int)std::max<int64_t>(1, N - 1);
+  std::vector<int2> sched((size_t)npairs * nrounds);
+  std::vector<int> ring(N);
+  for (int64_t i = 0; i < N; ++i) ring[i] = (int)i;
+  for (int r = 0; r < nrounds; ++r) {
+    for (int k = 0; k < npairs; ++k) {
+      int p = ring[k], q = ring[N - 1 - k];
+      if (p > q) std::swap(p, q);
+      sched[(size_t)r * npairs + k] = make_int2(p, q);
+    }
+    // keep ring[0], rotate the rest by one
+    const int last = ring[N - 1];
+    for (int64_t i = N - 1; i > 1; --i) ring[i] = ring[i - 1];
+    if (N > 1) ring[1] = last;
+  }
+  double2* M = nullptr;
+  int2* dpairs = nullptr;
+  Rot* rot = nullptr;
+  double* w = nullptr;
+  if ((rc = alloc(&M, (uint64_t)n * n))) return rc;
+  auto done = [&](int code) {
+    hipFree(M);
+    if (dpairs) hipFree(dpairs);
+    if (rot) hipFree(rot);
+    if (w) hipFree(w);
+    return code;
+  };
+  if (hipMalloc(&dpairs, sched.size() * sizeof(int2)) != hipSuccess ||
+      hipMalloc(&rot, (size_t)npairs * sizeof(Rot)) != hipSuccess || hipMalloc(&w, (size_t)n * sizeof(double)) != hipSuccess)
+    return done(api_fail(QSV_E_NOMEM, "eigen-hermitian scratch allocation failed"));
+  if (hipMemcpyAsync(dpairs, sched.data(), sched.size() * sizeof(int2), hipMemcpyHostToDevice, g_la_stream) != hipSuccess ||
+      hipMemcpyAsync(M, A->data, (size_t)n * n * sizeof(double2), hipMemcpyDeviceToDevice, g_la_stream) != hipSuccess)
+    return done(api_fail(QSV_E_DEVICE, "eigen-hermitian setup copy failed"));
+  hipLaunchKernelGGL(k_identity, dim3(grid_for((uint64_t)n * n)), dim3(kLaBlock), 0, g_la_stream, V->data, n);
+  int sweeps = 0;
+  const int kMaxSweeps = 40;
+  double r[2], prev_off = -1.0;
+  if (n > 1) {
+    for (; sweeps < kMaxSweeps; ++sweeps) {
+      if ((rc = reduce(R_OFFNORM, M, nullptr, n, n, r))) return done(rc);
+      // stop when the off-diagonal mass is negligible (1e-15 of the norm) or a sweep no longer reduces it
+      if (!(r[0] > 1e-30 * r[1]) || r[0] == 0.0 || (prev_off >= 0.0 && !(r[0] < prev_off))) break;
+      prev_off = r[0];
+      for (int rr = 0; rr < nrounds; ++rr) {
+        const int2* pr = dpairs + (size_t)rr * npairs;
+        hipLaunchKernelGGL(k_jac_params, dim3((npairs + kLaBlock - 1) / kLaBlock), dim3(kLaBlock), 0, g_la_stream,
+                           (const double2*)M, n, pr, npairs, rot);
+        hipLaunchKernelGGL(k_jac_rows, dim3(grid_for((uint64_t)npairs * n)), dim3(kLaBlock), 0, g_la_stream, M, n,
+                           pr, npairs, (const Rot*)rot);
+        hipLaunchKernelGGL(k_jac_cols, dim3(grid_for((uint64_t)npairs * n)), dim3(kLaBlock), 0, g_la_stream, M,
+                           V->data, n, pr, npairs, (const Rot*)rot);
+      }
+      if (hipGetLastError() != hipSuccess) return done(api_fail(QSV_E_DEVICE, "Jacobi launch failed"));
+    }
+    if (sweeps == kMaxSweeps) return done(api_fail(QSV_E_DEVICE, "eigen-hermitian: Jacobi did not converge"));
+  }
+  hipLaunchKernelGGL(k_diag_real, dim3(grid_for((uint64_t)n)), dim3(kLaBlock), 0, g_la_stream, (const double2*)M,
+                     n, w);
+  if (hipMemcpyAsync(eigenvalues, w, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, g_la_stream) != hipSuccess ||
+      hipStreamSynchronize(g_la_stream) != hipSuccess)
+    return done(api_fail(QSV_E_DEVICE, "eigen-hermitian readback failed"));
+  if (sweeps_out) *sweeps_out = sweeps;
+  return done(QSV_OK);
+}
+
+int qsv_la_kron_apply(const double* const* factors, int nfactors, qsv_mat* x)
+{
+  int rc = check_mat(x, "x");
+  if (rc) return rc;
+  if (nfactors < 1 || nfactors > 40 || !factors) return api_fail(QSV_E_INVALID, "kron-apply: bad factor list");
+  const uint64_t S = (uint64_t)x->rows * x->cols;
+  if (S != (1ull << nfactors)) return api_fail(QSV_E_INVALID, "kron-apply: vector length is not 2^factors");
+  std::lock_guard<std::mutex> lk(g_la_mu);
+  for (int k = 0; k < nfactors; ++k) {
+    const double* g = factors[k];
+    if (!g) continue;   // identity factor
+    const int b = nfactors - 1 - k;   // factor k acts on reference qubit k = index bit n-1-k
+    hipLaunchKernelGGL(k_apply2, dim3(grid_for(S / 2)), dim3(kLaBlock), 0, g_la_stream, x->data, S / 2, b,
+                       make_double2(g[0], g[1]), make_double2(g[2], g[3]), make_double2(g[4], g[5]),
+                       make_double2(g[6], g[7]));
+  }
+  LACHK(hipGetLastError());
+  LACHK(hipStreamSynchronize(g_la_stream));
+  return QSV_OK;
+}
+
+}  // extern "C"

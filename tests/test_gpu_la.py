@@ -1,0 +1,229 @@
+"""The :hip math backend (qclojure_amd/la.py over qsv_la.hip) against the
+linear-algebra oracle: the reference's math/backend_test.clj cases, random
+shapes that leave partial tiles at every edge of the MFMA GEMM, the lazy
+Kronecker path of the reference's gate expansion, Gauss-Jordan solve/inverse
+and the Jacobi eigensolver."""
+import sys
+
+import numpy as np
+import pytest
+
+import la_oracle as lo
+from qclojure_amd import circuit as qc
+from qclojure_amd import la
+
+pytestmark = pytest.mark.gpu
+
+I2 = np.eye(2, dtype=complex)
+CM = np.array([[1, -1j], [1j, 1]], dtype=complex)
+PZ = np.array([[1, 0], [0, -1]], dtype=complex)
+
+
+def note(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from qclojure_amd import engine
+    engine.init(0)
+    return la.create_backend("hip")
+
+
+def rand_c(rng, *shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def A_(x):
+    return np.asarray(x)
+
+
+def close(a, b, rel=4e-16):
+    """Equal up to the last bit of a product (libm/numpy may fuse where the device does not)."""
+    a, b = np.asarray(a), np.asarray(b)
+    return a.shape == b.shape and bool(np.all(np.abs(a - b) <= rel * np.maximum(np.abs(b), 1e-300)))
+
+
+def test_backend_test_cases(hip):
+    note("la: backend_test cases")
+    assert hip.shape([[1.0, 2.0, 3.0], [4.0, 5.0, 6.0], [7.0, 8.0, 9.0]]) == [3, 3]
+    assert hip.shape(CM) == [2, 2]
+    assert A_(hip.add(I2, I2))[0, 0] == 2
+    assert A_(hip.subtract(I2, I2))[0, 0] == 0
+    assert A_(hip.scale(I2, 2.0))[0, 0] == 2
+    assert A_(hip.negate(I2))[0, 0] == -1
+    assert np.array_equal(A_(hip.matrix_multiply(I2, CM)), CM)
+    assert np.allclose(A_(hip.matrix_vector_product(CM, [1, 1j]))[:, 0], lo.matrix_vector_product(CM, [1, 1j]))
+    assert hip.inner_product([1, 1j], [1, 1]) == lo.inner_product([1, 1j], [1, 1])
+    assert np.array_equal(A_(hip.outer_product([1, 1j], [1, 1])), lo.outer_product([1, 1j], [1, 1]))
+    assert hip.shape(hip.kronecker_product(I2, I2)) == [4, 4]
+    assert np.array_equal(A_(hip.transpose(CM)), CM.T)
+    assert np.array_equal(A_(hip.conjugate_transpose(CM)), CM)
+    assert hip.trace(CM) == lo.trace(CM)
+    assert hip.norm2([3.0, 4.0]) == 5.0
+    assert hip.hermitian(CM) is True and hip.unitary(I2) is True
+    e = hip.eigen_hermitian(PZ)
+    assert e["eigenvalues"] == [-1, 1]
+    for lam, v in zip(e["eigenvalues"], e["eigenvectors"]):
+        assert np.linalg.norm(PZ @ v - lam * v) < 1e-10
+    # SoA and complex-map inputs (fastmath/backend.clj:132-147)
+    soa = {"real": [[1, 2], [3, 4]], "imag": [[5, 6], [7, 8]]}
+    assert A_(hip.matrix_to_backend(soa))[1, 1] == 4 + 8j
+    cm = [[{"real": 1.0, "imag": 0.0}, {"real": 0.0, "imag": -1.0}], [{"real": 0.0, "imag": 1.0}, 1.0]]
+    assert np.array_equal(A_(hip.matrix_to_backend(cm)), CM)
+
+
+@pytest.mark.parametrize("m,k,n", [(1, 1, 1), (2, 3, 5), (17, 33, 65), (64, 64, 64), (100, 7, 129),
+                                   (257, 300, 129), (513, 64, 70)])
+def test_gemm_shapes(hip, m, k, n):
+    note(f"la: gemm {m}x{k}x{n}")
+    rng = np.random.default_rng(m * 1000 + k * 10 + n)
+    A, B = rand_c(rng, m, k), rand_c(rng, k, n)
+    got = A_(hip.matrix_multiply(A, B))
+    assert np.max(np.abs(got - A @ B)) < 1e-12 * max(1, k)
+    # conjugate-transpose operand (as unitary? uses it)
+    Ad = hip.matrix_to_backend(A)               # m x k, op = A^H gives k x m
+    Bd = hip.matrix_to_backend(rand_c(rng, m, n))
+    Cd = la.DeviceMatrix(k, n)
+    la._call(la._lib().qsv_la_gemm, 1, Ad.handle, Bd.handle, Cd.handle)
+    assert np.max(np.abs(Cd.to_numpy() - A.conj().T @ Bd.to_numpy())) < 1e-12 * max(1, m)
+
+
+def test_gemv_and_elementwise(hip):
+    note("la: gemv, elementwise, transposes")
+    rng = np.random.default_rng(5)
+    for m, n in ((1, 1), (3, 70), (200, 129), (1000, 1000)):
+        A, x = rand_c(rng, m, n), rand_c(rng, n)
+        assert np.max(np.abs(A_(hip.matrix_vector_product(A, x))[:, 0] - A @ x)) < 1e-12 * n
+        B = rand_c(rng, m, n)
+        assert np.array_equal(A_(hip.add(A, B)), A + B)
+        assert np.array_equal(A_(hip.subtract(A, B)), A - B)
+        assert close(A_(hip.scale(A, 0.5 - 2j)), A * (0.5 - 2j))
+        assert close(A_(hip.hadamard_product(A, B)), lo.hadamard_product(A, B))
+        assert np.array_equal(A_(hip.transpose(A)), A.T)
+        assert np.array_equal(A_(hip.conjugate_transpose(A)), A.conj().T)
+        y = rand_c(rng, m)
+        assert close(A_(hip.outer_product(y, x)), lo.outer_product(y, x))
+    x, y = rand_c(rng, 100003), rand_c(rng, 100003)
+    assert abs(hip.inner_product(x, y) - np.vdot(x, y)) < 1e-9
+    assert abs(hip.norm2(x) - np.linalg.norm(x)) < 1e-9
+    M = rand_c(rng, 300, 300)
+    assert abs(hip.trace(M) - np.trace(M)) < 1e-11
+    with pytest.raises(la.LinearAlgebraError):
+        hip.trace(rand_c(rng, 2, 3))
+    with pytest.raises(la.LinearAlgebraError):
+        hip.add(rand_c(rng, 2, 3), rand_c(rng, 3, 2))
+
+
+def test_kronecker_dense_and_lazy(hip, oracle):
+    note("la: kronecker")
+    rng = np.random.default_rng(9)
+    A, B = rand_c(rng, 3, 2), rand_c(rng, 4, 5)
+    K = hip.kronecker_product(A, B)
+    assert close(A_(K), lo.kronecker_product(A, B))
+    # the reference's expand-gate-to-n-qubits: (reduce kronecker-product [I .. G .. I])
+    n, t = 12, 5
+    G = np.array([[0.6, 0.8j], [0.8j, 0.6]], complex)
+    E = I2
+    for q in range(1, n):
+        E = hip.kronecker_product(E, G if q == t else I2)
+    assert isinstance(E, la.KronMatrix) and hip.shape(E) == [1 << n, 1 << n]
+    psi = rand_c(rng, 1 << n)
+    got = A_(hip.matrix_vector_product(E, psi))[:, 0]
+    # against the engine's own gate on the same state, and the dense product on a smaller case
+    ops = qc.ops_from_tuples([(qc.OP["U"], t, 0, 0, 0, 0.0)])
+    ops["m"][0] = [G[0, 0].real, G[0, 0].imag, G[0, 1].real, G[0, 1].imag,
+                   G[1, 0].real, G[1, 0].imag, G[1, 1].real, G[1, 1].imag]
+    exp = oracle.apply_program(psi, n, ops)
+    assert np.max(np.abs(got - exp)) < 1e-12
+    small = hip.kronecker_product(hip.kronecker_product(I2, G), I2)
+    v = rand_c(rng, 8)
+    assert np.max(np.abs(A_(hip.matrix_vector_product(small, v))[:, 0] - np.kron(np.kron(I2, G), I2) @ v)) < 1e-14
+    # mixed-product rule keeps products of expansions lazy
+    P = hip.matrix_multiply(small, small)
+    assert isinstance(P, la.KronMatrix)
+    assert np.allclose(A_(P), np.kron(np.kron(I2, G @ G), I2), atol=1e-15)
+
+
+def test_predicates(hip):
+    note("la: predicates")
+    rng = np.random.default_rng(3)
+    H = rand_c(rng, 50, 50)
+    H = H + H.conj().T
+    assert hip.hermitian(H) and not hip.hermitian(H + 1e-9 * np.eye(50) * 1j)
+    assert hip.diagonal(np.diag(rand_c(rng, 40))) and not hip.diagonal(np.diag(rand_c(rng, 40)) + 1e-9 * np.eye(40, k=3))
+    Q, _ = np.linalg.qr(rand_c(rng, 64, 64))
+    assert hip.unitary(Q) is lo.unitary(Q, 1e-12)
+    assert hip.unitary(Q, 1e-6) and not hip.unitary(2 * Q, 1e-6)
+    assert hip.positive_semidefinite(np.array([[2, 1], [1, 2]], complex))
+    assert not hip.positive_semidefinite(np.array([[1, 2], [2, 1]], complex))
+    with pytest.raises(la.LinearAlgebraError):
+        hip.positive_semidefinite(np.array([[1, 2], [3, 1]], complex))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 16, 65, 200])
+def test_solve_and_inverse(hip, n):
+    note(f"la: solve/inverse n={n}")
+    rng = np.random.default_rng(40 + n)
+    A = rand_c(rng, n, n) + 2 * np.sqrt(n) * np.eye(n)
+    b = rand_c(rng, n)
+    x = A_(hip.solve_linear_system(A, b))[:, 0]
+    assert np.max(np.abs(x - lo.solve_linear(A, b))) < 1e-11
+    Ai = A_(hip.inverse(A))
+    assert np.max(np.abs(Ai - lo.inverse(A))) < 1e-11
+    assert np.max(np.abs(Ai @ A - np.eye(n))) < 1e-11
+
+
+def test_singular_and_pivoting(hip):
+    note("la: singular")
+    for bad in (np.array([[1, 2], [2, 4]], complex), np.zeros((3, 3), complex)):
+        with pytest.raises(la.LinearAlgebraError, match="singular"):
+            hip.inverse(bad)
+        with pytest.raises(la.LinearAlgebraError, match="singular"):
+            hip.solve_linear_system(bad, np.ones(bad.shape[0]))
+    P = np.array([[0, 1, 0], [1, 0, 0], [-1, 0, 1]], complex)   # needs row exchanges
+    assert np.array_equal(A_(hip.inverse(P)), lo.inverse(P))
+    with pytest.raises(la.LinearAlgebraError):
+        hip.inverse(rand_c(np.random.default_rng(0), 2, 3))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 64, 101, 256])
+def test_eigen_hermitian(hip, n):
+    note(f"la: eigh n={n}")
+    rng = np.random.default_rng(70 + n)
+    H = rand_c(rng, n, n)
+    H = H + H.conj().T
+    e = hip.eigen_hermitian(H)
+    w = np.array([v.real for v in e["eigenvalues"]])
+    ref = np.linalg.eigvalsh(H)
+    scale = max(1.0, np.max(np.abs(ref)))
+    assert np.all(np.diff(w) >= 0)
+    assert np.max(np.abs(w - ref)) < 1e-11 * scale
+    V = np.array(e["eigenvectors"]).T
+    assert np.max(np.abs(H @ V - V * w)) < 1e-10 * scale
+    assert np.max(np.abs(V.conj().T @ V - np.eye(n))) < 1e-11
+    # real symmetric input: the reference's real branch, eigenvalues as the oracle's
+    S = rng.standard_normal((n, n))
+    S = S + S.T
+    e = hip.eigen_hermitian(S)
+    assert np.max(np.abs(np.array([v.real for v in e["eigenvalues"]]) -
+                         np.array([v.real for v in lo.eigen_hermitian(S)["eigenvalues"]]))) < 1e-11 * max(1, n)
+
+
+def test_degenerate_spectrum_and_state_view(hip):
+    note("la: degenerate eigh, state view")
+    from qclojure_amd.engine import DeviceState
+    X = np.array([[0, 1], [1, 0]], complex)
+    H = np.kron(np.kron(X, I2), X) + np.kron(np.kron(I2, PZ), I2)   # eigenvalues in {-2, 0, 2}, degenerate
+    e = hip.eigen_hermitian(H)
+    w = np.array([v.real for v in e["eigenvalues"]])
+    assert np.allclose(w, np.linalg.eigvalsh(H), atol=1e-12)
+    V = np.array(e["eigenvectors"]).T
+    assert np.max(np.abs(H @ V - V * w)) < 1e-12
+    with DeviceState(6) as st:
+        st.init_basis(5)
+        view = la.DeviceMatrix.view_state(st)
+        assert hip.norm2(view) == 1.0
+        amps = A_(view)[:, 0]
+        assert amps[5] == 1 and np.count_nonzero(amps) == 1
+        view.close()
