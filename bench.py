@@ -366,7 +366,7 @@ def main():
     ap.add_argument("--serve", type=int, default=120, help="served requests per arrival rate (0 to skip)")
     ap.add_argument("--serve-plain", type=int, default=0,
                     help="requests timed on the plain path (0: the same requests as served)")
-    ap.add_argument("--rates", default="5,10,20,50,1000")
+    ap.add_argument("--rates", default="50,200,1000,2000")
     ap.add_argument("--precision-qubits", type=int, default=32, help="0 to skip the precision study")
     ap.add_argument("--precision-layers", type=int, default=50)
     ap.add_argument("--la", type=int, default=4096, help="GEMM size of the linear-algebra section (0 to skip)")

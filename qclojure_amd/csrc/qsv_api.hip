@@ -1379,17 +1379,55 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
   for (uint64_t i = 0; i < N; ++i) {
     const int n = d->num_qubits[i];
     if (n < 1 || n > 34) return fail(QSV_E_INVALID, "instance " + std::to_string(i) + ": num_qubits out of range");
-    std::vector<BGate> gs;
-    uint64_t mk = 0;
-    for (uint64_t k = d->op_offsets[i]; k < d->op_offsets[i + 1]; ++k) {
-      if (d->ops[k].kind == QSV_OP_MEASURE)
-        return fail(QSV_E_UNSUPPORTED, "instance " + std::to_string(i) + ": measurements are not batched");
-      rc = decode_op(n, d->ops[k], 0, 0, &mk, gs);
-      if (rc) {
-        g_err = "instance " + std::to_string(i) + ", op " + std::to_string(k - d->op_offsets[i]) + ": " + g_err;
-        return rc;
+  }
+  // Decode every instance, then fuse the programs of the instances too large
+  // for LDS into grouped passes: independent per instance, so both run on a
+  // pool of host threads (at most 16); errors are reported for the first
+  // failing instance in order, as a sequential decode would.
+  std::vector<std::vector<BGate>> decoded(N);
+  struct Plan {
+    std::vector<char> bytes;
+    std::vector<int64_t> offs;
+    bool ok = false;
+  };
+  std::vector<Plan> plans(N);
+  std::vector<int> dec_rc(N, 0);
+  std::vector<std::string> dec_err(N);
+  {
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const unsigned nt = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(hw, N / 8 + 1));
+    std::atomic<uint64_t> next{0};
+    auto worker = [&]() {
+      for (uint64_t i; (i = next.fetch_add(1)) < N;) {
+        const int n = d->num_qubits[i];
+        uint64_t mk = 0;
+        for (uint64_t k = d->op_offsets[i]; k < d->op_offsets[i + 1]; ++k) {
+          if (d->ops[k].kind == QSV_OP_MEASURE) {
+            dec_rc[i] = QSV_E_UNSUPPORTED;
+            dec_err[i] = "instance " + std::to_string(i) + ": measurements are not batched";
+            break;
+          }
+          const int r = decode_op(n, d->ops[k], 0, 0, &mk, decoded[i]);
+          if (r) {
+            dec_rc[i] = r;
+            dec_err[i] = "instance " + std::to_string(i) + ", op " + std::to_string(k - d->op_offsets[i]) + ": " + g_err;
+            break;
+          }
+        }
+        if (!dec_rc[i] && n > lds_n) plans[i].ok = plan_group_passes(n, prec, decoded[i], plans[i].bytes, plans[i].offs);
       }
-    }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < nt; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto& t : pool) t.join();
+  }
+  for (uint64_t i = 0; i < N; ++i)
+    if (dec_rc[i]) return fail(dec_rc[i], dec_err[i]);
+  std::vector<Plan> big_plans;
+  for (uint64_t i = 0; i < N; ++i) {
+    const int n = d->num_qubits[i];
+    std::vector<BGate>& gs = decoded[i];
     BInst& in = insts[i];
     in.n = n;
     in.t0 = tz.size();
@@ -1424,6 +1462,7 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
       in.g0 = in.g1 = 0;
       big_idx.push_back(i);
       big_gates.push_back(std::move(gs));
+      big_plans.push_back(std::move(plans[i]));
     }
   }
   // largest circuits first, so the long workgroups start early
@@ -1516,13 +1555,16 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
         std::vector<std::vector<int64_t>> offs_of;
         std::vector<size_t> chunk;
         for (size_t m = c0; m < members.size() && m < c0 + per_chunk; ++m) {
-          std::vector<int64_t> offs;
-          const size_t mark = bytes.size();
-          if (plan_group_passes(n, prec, big_gates[members[m]], bytes, offs)) {
+          Plan& pl = big_plans[members[m]];
+          if (pl.ok) {
+            // per-instance descriptors are 256-byte multiples: appending keeps them aligned
+            const int64_t base = (int64_t)bytes.size();
+            bytes.insert(bytes.end(), pl.bytes.begin(), pl.bytes.end());
+            std::vector<int64_t> offs(pl.offs);
+            for (int64_t& o : offs) o += base;
             chunk.push_back(members[m]);
             offs_of.push_back(std::move(offs));
           } else {
-            bytes.resize(mark);
             singles.push_back(members[m]);
           }
         }
