@@ -79,10 +79,11 @@ def position_sweep(engine, n: int, prec: str, reps: int = 3) -> dict:
              "T": lambda q: [(OP["T"], q, 0, 0, 0, 0.0)],
              "CNOT(c=q-1)": lambda q: [(OP["CNOT"], (q - 1) % n, q, 0, 0, 0.0)],
              "CZ(c=q-1)": lambda q: [(OP["CZ"], (q - 1) % n, q, 0, 0, 0.0)],
-             "SWAP(q,q-1)": lambda q: [(OP["SWAP"], q, (q - 1) % n, 0, 0, 0.0)]}
+             # SWAP's qubit numbers are LSB bit positions (gate.clj:768-778): this swaps index bits b and b-1
+             "SWAP(b,b-1)": lambda q: [(OP["SWAP"], n - 1 - q, n - 2 - q, 0, 0, 0.0)]}
     for name, mk in gates.items():
         row = {}
-        for q in range(n):
+        for q in range(n - 1 if name.startswith("SWAP") else n):
             ops = qc.ops_from_tuples(mk(q) * reps)
             dev.apply(ops[:1])
             dev.sync()

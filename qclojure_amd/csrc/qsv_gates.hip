@@ -188,11 +188,11 @@ __global__ __launch_bounds__(kBlock) void k_phase(typename NV<R>::T* __restrict_
 // Swap of two bits A, B (phase ph on the exchanged amplitudes), optionally
 // controlled. MODE 0: both high; 1: hA high, lB low; 2: both low.
 // ---------------------------------------------------------------------------
-template <typename R, int PPT, int MODE>
+template <typename R, int PPT, int MODE, bool XCD>
 __global__ __launch_bounds__(kBlock) void k_swap2(typename NV<R>::T* __restrict__ psi, SwapArgs s, Cplx<double> ph)
 {
   using T = typename NV<R>::T;
-  const uint64_t base = (uint64_t)blockIdx.x * (kBlock * PPT) + threadIdx.x;
+  const uint64_t base = block_chunk<XCD>() * (kBlock * PPT) + threadIdx.x;
   const uint64_t count = s.ins.count;
   uint64_t id[PPT];
   T a[PPT], b[PPT];
@@ -342,17 +342,31 @@ void launch_swap2(int prec, void* psi, const SwapArgs& s, const double ph[2], hi
   if (s.ins.count == 0) return;
   const Cplx<double> p{ph[0], ph[1]};
   const int mode = (s.hA && s.hB) ? 0 : (s.hA ? 1 : 2);
-#define QSV_SWAP_LAUNCH(MODE, PPT)                                                                              \
+  // the pair kernel's policy: partners at least 4 MiB apart stream best with
+  // four pairs per thread in plain order, closer ones with two pairs per
+  // thread in XCD-contiguous order
+  const uint64_t B = prec == QSV_C128 ? 16 : 8;
+  const uint64_t dist = mode == 0 ? (s.hA > s.hB ? s.hA - s.hB : s.hB - s.hA) : s.hA;
+  const bool far = mode != 2 && dist * B >= (4ull << 20);
+#define QSV_SWAP_LAUNCH(MODE, PPT, XCD)                                                                         \
   do {                                                                                                          \
     const unsigned grid = blocks_for(s.ins.count, PPT);                                                         \
     if (prec == QSV_C128)                                                                                       \
-      hipLaunchKernelGGL((k_swap2<double, PPT, MODE>), dim3(grid), dim3(kBlock), 0, st, (NV<double>::T*)psi, s, p); \
+      hipLaunchKernelGGL((k_swap2<double, PPT, MODE, XCD>), dim3(grid), dim3(kBlock), 0, st, (NV<double>::T*)psi, \
+                         s, p);                                                                                 \
     else                                                                                                        \
-      hipLaunchKernelGGL((k_swap2<float, PPT, MODE>), dim3(grid), dim3(kBlock), 0, st, (NV<float>::T*)psi, s, p);   \
+      hipLaunchKernelGGL((k_swap2<float, PPT, MODE, XCD>), dim3(grid), dim3(kBlock), 0, st, (NV<float>::T*)psi,   \
+                         s, p);                                                                                 \
   } while (0)
-  if (mode == 0) QSV_SWAP_LAUNCH(0, 2);
-  else if (mode == 1) QSV_SWAP_LAUNCH(1, 2);
-  else QSV_SWAP_LAUNCH(2, 4);
+  if (mode == 0) {
+    if (far) QSV_SWAP_LAUNCH(0, 4, false);
+    else QSV_SWAP_LAUNCH(0, 2, true);
+  } else if (mode == 1) {
+    if (far) QSV_SWAP_LAUNCH(1, 4, false);
+    else QSV_SWAP_LAUNCH(1, 2, true);
+  } else {
+    QSV_SWAP_LAUNCH(2, 4, true);
+  }
 #undef QSV_SWAP_LAUNCH
 }
 
