@@ -205,7 +205,12 @@ __global__ __launch_bounds__(1 << (TB - kRegBits), 3) void k_tile(typename FV<R>
 {
   using T = typename FV<R>::T;
   constexpr uint32_t FULL = (1u << TB) - 1u;
-  __shared__ T lds[1 << TB];
+  // complex128 tiles go through LDS one plane (real, then imaginary) at a
+  // time: 32 KiB instead of 64, so three workgroups fit a CU
+  constexpr bool SPLIT = sizeof(T) == 16;
+  __shared__ __align__(16) char smem[SPLIT ? (sizeof(R) << TB) : (sizeof(T) << TB)];
+  T* const lds = reinterpret_cast<T*>(smem);
+  R* const ldp = reinterpret_cast<R*>(smem);
   __shared__ double red[(1 << (TB - kRegBits)) / 64];
   if constexpr (GROUP) {
     const int64_t off = offs[blockIdx.y];
@@ -259,6 +264,35 @@ __global__ __launch_bounds__(1 << (TB - kRegBits), 3) void k_tile(typename FV<R>
   };
 
   T x[kRegs];
+  // move the tile from layout `from` to layout `to` through LDS
+  auto relayout = [&](uint32_t from, uint32_t to) {
+    if constexpr (SPLIT) {
+#pragma unroll
+      for (int k = 0; k < kRegs; ++k) ldp[swz(kl(k))] = x[k].x;
+      __syncthreads();
+      set_layout(to);
+#pragma unroll
+      for (int k = 0; k < kRegs; ++k) x[k].x = ldp[swz(kl(k))];
+      __syncthreads();
+      set_layout(from);
+#pragma unroll
+      for (int k = 0; k < kRegs; ++k) ldp[swz(kl(k))] = x[k].y;
+      __syncthreads();
+      set_layout(to);
+#pragma unroll
+      for (int k = 0; k < kRegs; ++k) x[k].y = ldp[swz(kl(k))];
+      __syncthreads();
+    } else {
+#pragma unroll
+      for (int k = 0; k < kRegs; ++k) lds[swz(kl(k))] = x[k];
+      __syncthreads();
+      set_layout(to);
+#pragma unroll
+      for (int k = 0; k < kRegs; ++k) x[k] = lds[swz(kl(k))];
+      __syncthreads();
+    }
+  };
+
   set_layout(reg);
   {
     uint64_t ga[kRegs];
@@ -285,14 +319,8 @@ __global__ __launch_bounds__(1 << (TB - kRegBits), 3) void k_tile(typename FV<R>
   for (int s = 0; s < nst; ++s) {
     const FStage st = pass->stages[s];
     if (st.regmask != reg) {
-#pragma unroll
-      for (int k = 0; k < kRegs; ++k) lds[swz(kl(k))] = x[k];
-      __syncthreads();
+      relayout(reg, st.regmask);
       reg = st.regmask;
-      set_layout(reg);
-#pragma unroll
-      for (int k = 0; k < kRegs; ++k) x[k] = lds[swz(kl(k))];
-      __syncthreads();
     }
     for (int gi = st.g0; gi < st.g1; ++gi) {
       const FGate* g = &pass->gates[gi];
@@ -345,16 +373,41 @@ __global__ __launch_bounds__(1 << (TB - kRegBits), 3) void k_tile(typename FV<R>
         // every thread writes its slots to permuted LDS addresses
         const uint32_t ba = 1u << g->a, bb = 1u << g->b;
         const A pr = (A)g->m[0], pi = (A)g->m[1];
+        if constexpr (SPLIT) {
+          // real plane, then imaginary plane; each slot goes to its partner's address
 #pragma unroll
-        for (int k = 0; k < kRegs; ++k) {
-          const uint32_t l = kl(k);
-          const bool mv = ((l & cm) == cm) && (((l & ba) != 0) != ((l & bb) != 0));
-          lds[swz(mv ? (l ^ ba ^ bb) : l)] = mv ? cscale<T, R, A>(pr, pi, x[k]) : x[k];
+          for (int k = 0; k < kRegs; ++k) {
+            const uint32_t l = kl(k);
+            const bool mv = ((l & cm) == cm) && (((l & ba) != 0) != ((l & bb) != 0));
+            if (mv) x[k] = cscale<T, R, A>(pr, pi, x[k]);
+            ldp[swz(mv ? (l ^ ba ^ bb) : l)] = x[k].x;
+          }
+          __syncthreads();
+#pragma unroll
+          for (int k = 0; k < kRegs; ++k) x[k].x = ldp[swz(kl(k))];
+          __syncthreads();
+#pragma unroll
+          for (int k = 0; k < kRegs; ++k) {
+            const uint32_t l = kl(k);
+            const bool mv = ((l & cm) == cm) && (((l & ba) != 0) != ((l & bb) != 0));
+            ldp[swz(mv ? (l ^ ba ^ bb) : l)] = x[k].y;
+          }
+          __syncthreads();
+#pragma unroll
+          for (int k = 0; k < kRegs; ++k) x[k].y = ldp[swz(kl(k))];
+          __syncthreads();
+        } else {
+#pragma unroll
+          for (int k = 0; k < kRegs; ++k) {
+            const uint32_t l = kl(k);
+            const bool mv = ((l & cm) == cm) && (((l & ba) != 0) != ((l & bb) != 0));
+            lds[swz(mv ? (l ^ ba ^ bb) : l)] = mv ? cscale<T, R, A>(pr, pi, x[k]) : x[k];
+          }
+          __syncthreads();
+#pragma unroll
+          for (int k = 0; k < kRegs; ++k) x[k] = lds[swz(kl(k))];
+          __syncthreads();
         }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < kRegs; ++k) x[k] = lds[swz(kl(k))];
-        __syncthreads();
       } else {  // FG_BLOCKADE: phase where exactly one bit of bmask is set
         const uint32_t bm = g->bmask;
         const A pr = (A)g->m[0], pi = (A)g->m[1];
@@ -366,13 +419,8 @@ __global__ __launch_bounds__(1 << (TB - kRegBits), 3) void k_tile(typename FV<R>
   }
 
   if (reg != pass->store_mask) {
-#pragma unroll
-    for (int k = 0; k < kRegs; ++k) lds[swz(kl(k))] = x[k];
-    __syncthreads();
+    relayout(reg, pass->store_mask);
     reg = pass->store_mask;
-    set_layout(reg);
-#pragma unroll
-    for (int k = 0; k < kRegs; ++k) x[k] = lds[swz(kl(k))];
   }
   {
     uint64_t ga[kRegs];
