@@ -199,7 +199,7 @@ __device__ __forceinline__ double norm2_slots(const T (&x)[kRegs])
 // negative offset: no pass this round). Pass descriptors of a group are stored
 // truncated after their last gate. Group passes do not renormalise.
 template <typename R, typename A, int TB, bool GROUP>
-__global__ __launch_bounds__(1 << (TB - kRegBits), 3) void k_tile(typename FV<R>::T* __restrict__ psi,
+__global__ __launch_bounds__(1 << (TB - kRegBits), (sizeof(R) == 4 && sizeof(A) == 4) ? 4 : 3) void k_tile(typename FV<R>::T* __restrict__ psi,
                                                                const FPass* __restrict__ pass, TileNorm nb,
                                                                uint64_t S, const int64_t* __restrict__ offs)
 {
