@@ -170,7 +170,7 @@ constexpr int GBM = 64, GBN = 64, GBK = 16;
 // A[l % 16][l / 16] and B[l / 16][l % 16]; it receives C[4 r + l / 16][l % 16]
 // in accumulator element r (rows interleaved across the four lane groups).
 template <bool CONJA>
-__global__ __launch_bounds__(256) void k_zgemm(const double2* __restrict__ A, const double2* __restrict__ B,
+__global__ __launch_bounds__(256, 3) void k_zgemm(const double2* __restrict__ A, const double2* __restrict__ B,
                                                double2* __restrict__ C, int M, int N, int K)
 {
   __shared__ double sAr[GBK][GBM + 1], sAi[GBK][GBM + 1];
@@ -186,28 +186,45 @@ __global__ __launch_bounds__(256) void k_zgemm(const double2* __restrict__ A, co
       accr[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
       acci[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
     }
-  for (int k0 = 0; k0 < K; k0 += GBK) {
+  // The next K chunk is fetched into registers (4 elements of A and 4 of B
+  // per thread) while the MFMAs consume the current one from LDS.
+  double2 ra[4], rb[4];
+  auto fetch = [&](int k0) {
 #pragma unroll
-    for (int e = tid; e < GBM * GBK; e += 256) {
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q;
       int ii, kk;
       if (CONJA) { kk = e / GBM; ii = e % GBM; }
       else { ii = e / GBK; kk = e % GBK; }
       const int gi = i0 + ii, gk = k0 + kk;
-      double2 v = make_double2(0.0, 0.0);
-      if (gi < M && gk < K) v = CONJA ? A[(size_t)gk * M + gi] : A[(size_t)gi * K + gk];
-      sAr[kk][ii] = v.x;
-      sAi[kk][ii] = CONJA ? -v.y : v.y;
+      ra[q] = make_double2(0.0, 0.0);
+      if (gi < M && gk < K) ra[q] = CONJA ? A[(size_t)gk * M + gi] : A[(size_t)gi * K + gk];
+      const int kb = e / GBN, jb = e % GBN;
+      const int gkb = k0 + kb, gj = j0 + jb;
+      rb[q] = make_double2(0.0, 0.0);
+      if (gkb < K && gj < N) rb[q] = B[(size_t)gkb * N + gj];
     }
+  };
+  auto stage = [&]() {
 #pragma unroll
-    for (int e = tid; e < GBK * GBN; e += 256) {
-      const int kk = e / GBN, jj = e % GBN;
-      const int gk = k0 + kk, gj = j0 + jj;
-      double2 v = make_double2(0.0, 0.0);
-      if (gk < K && gj < N) v = B[(size_t)gk * N + gj];
-      sBr[kk][jj] = v.x;
-      sBi[kk][jj] = v.y;
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q;
+      int ii, kk;
+      if (CONJA) { kk = e / GBM; ii = e % GBM; }
+      else { ii = e / GBK; kk = e % GBK; }
+      sAr[kk][ii] = ra[q].x;
+      sAi[kk][ii] = CONJA ? -ra[q].y : ra[q].y;
+      const int kb = e / GBN, jb = e % GBN;
+      sBr[kb][jb] = rb[q].x;
+      sBi[kb][jb] = rb[q].y;
     }
-    __syncthreads();
+  };
+  fetch(0);
+  stage();
+  __syncthreads();
+  for (int k0 = 0; k0 < K; k0 += GBK) {
+    const bool more = k0 + GBK < K;
+    if (more) fetch(k0 + GBK);
 #pragma unroll
     for (int ks = 0; ks < GBK; ks += 4) {
       const int kk = ks + (lane >> 4), r16 = lane & 15;
@@ -235,6 +252,10 @@ __global__ __launch_bounds__(256) void k_zgemm(const double2* __restrict__ A, co
         }
     }
     __syncthreads();
+    if (more) {
+      stage();
+      __syncthreads();
+    }
   }
 #pragma unroll
   for (int a = 0; a < 2; ++a)
