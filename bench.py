@@ -308,7 +308,7 @@ def bench_la(engine, n: int, seed: int) -> dict:
     out["zgemv"] = {"n": m, "ms": round(ms, 3), "GB_s": round(gbs, 1), "frac_of_8TB_s": round(gbs / 8000.0, 4)}
     G.close()
     x.close()
-    for k in (64, 256):
+    for k in (16, 64, 256):
         H = rng.standard_normal((k, k)) + 1j * rng.standard_normal((k, k))
         H = H + H.conj().T
         Hd = la.DeviceMatrix.from_numpy(H)

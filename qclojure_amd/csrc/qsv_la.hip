@@ -591,6 +591,123 @@ __global__ void k_identity(double2* __restrict__ V, int64_t n)
   }
 }
 
+// The whole Jacobi iteration for n <= kJacSmall in one workgroup, A and V in
+// LDS: the same rotations, round-robin schedule and stopping rule as the
+// multi-launch path, with barriers in place of launches.
+constexpr int kJacSmall = 32;
+
+__global__ __launch_bounds__(256) void k_jacobi_small(const double2* __restrict__ A, double2* __restrict__ Vout,
+                                                      double* __restrict__ w, int n, int* __restrict__ sweeps_out)
+{
+  __shared__ double2 sM[kJacSmall][kJacSmall];
+  __shared__ double2 sV[kJacSmall][kJacSmall];
+  __shared__ Rot rot[kJacSmall / 2];
+  __shared__ int2 pr[kJacSmall / 2];
+  __shared__ double ra[kLaBlock / 64], rb[kLaBlock / 64];
+  __shared__ int stop;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < n * n; e += blockDim.x) {
+    const int i = e / n, j = e - i * n;
+    sM[i][j] = A[e];
+    sV[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+  }
+  const int N = n + (n & 1), npairs = N / 2, nrounds = N - 1;
+  double prev = -1.0;
+  int sweeps = 0;
+  __syncthreads();
+  for (; sweeps < 40; ++sweeps) {
+    double off = 0.0, tot = 0.0;
+    for (int e = tid; e < n * n; e += blockDim.x) {
+      const int i = e / n, j = e - i * n;
+      const double m2 = cabs2(sM[i][j]);
+      tot += m2;
+      if (i != j) off += m2;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      off += __shfl_xor(off, o);
+      tot += __shfl_xor(tot, o);
+    }
+    if ((tid & 63) == 0) { ra[tid >> 6] = off; rb[tid >> 6] = tot; }
+    __syncthreads();
+    if (tid == 0) {
+      double o = 0.0, t = 0.0;
+      for (int i = 0; i < kLaBlock / 64; ++i) { o += ra[i]; t += rb[i]; }
+      stop = (!(o > 1e-30 * t) || o == 0.0 || (prev >= 0.0 && !(o < prev))) ? 1 : 0;
+      ra[0] = o;
+    }
+    __syncthreads();
+    if (stop) break;
+    prev = ra[0];
+    __syncthreads();
+    for (int r = 0; r < nrounds; ++r) {
+      if (tid < npairs) {
+        // circle method: position 0 fixed, the others rotated by r
+        auto player = [&](int pos) { return pos == 0 ? 0 : ((pos - 1 - r) % (N - 1) + (N - 1)) % (N - 1) + 1; };
+        int p = player(tid), q = player(N - 1 - tid);
+        if (p > q) { const int t = p; p = q; q = t; }
+        pr[tid] = make_int2(p, q);
+        Rot R{1.0, 0.0, 1.0, 0.0};
+        if (q < n) {
+          const double2 bu = sM[p][q], bl = sM[q][p];
+          const double2 b = make_double2(0.5 * (bu.x + bl.x), 0.5 * (bu.y - bl.y));
+          const double sc = fmax(fabs(b.x), fabs(b.y));
+          if (sc > 1e-290) {
+            const double bx = b.x / sc, by = b.y / sc;
+            const double rr = sqrt(bx * bx + by * by);
+            const double ab = sc * rr;
+            const double tau = (sM[q][q].x - sM[p][p].x) / (2.0 * ab);
+            const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+            const double c = 1.0 / sqrt(1.0 + t * t);
+            R = Rot{c, t * c, bx / rr, -by / rr};
+          }
+        }
+        rot[tid] = R;
+      }
+      __syncthreads();
+      for (int e = tid; e < npairs * n; e += blockDim.x) {
+        const int k = e / n, j = e - k * n;
+        const int p = pr[k].x, q = pr[k].y;
+        if (q >= n) continue;
+        const Rot R = rot[k];
+        const double2 eq = cmul(make_double2(R.er, -R.ei), sM[q][j]);
+        const double2 ap = sM[p][j];
+        sM[p][j] = make_double2(R.c * ap.x - R.s * eq.x, R.c * ap.y - R.s * eq.y);
+        sM[q][j] = make_double2(R.s * ap.x + R.c * eq.x, R.s * ap.y + R.c * eq.y);
+      }
+      __syncthreads();
+      for (int e = tid; e < npairs * n; e += blockDim.x) {
+        const int i = e / npairs, k = e - i * npairs;
+        const int p = pr[k].x, q = pr[k].y;
+        if (q >= n) continue;
+        const Rot R = rot[k];
+        const double2 E = make_double2(R.er, R.ei);
+        {
+          const double2 xp = sM[i][p], eq = cmul(E, sM[i][q]);
+          double2 np_ = make_double2(R.c * xp.x - R.s * eq.x, R.c * xp.y - R.s * eq.y);
+          double2 nq_ = make_double2(R.s * xp.x + R.c * eq.x, R.s * xp.y + R.c * eq.y);
+          if (i == p) { np_.y = 0.0; nq_ = make_double2(0.0, 0.0); }
+          if (i == q) { np_ = make_double2(0.0, 0.0); nq_.y = 0.0; }
+          sM[i][p] = np_;
+          sM[i][q] = nq_;
+        }
+        {
+          const double2 xp = sV[i][p], eq = cmul(E, sV[i][q]);
+          sV[i][p] = make_double2(R.c * xp.x - R.s * eq.x, R.c * xp.y - R.s * eq.y);
+          sV[i][q] = make_double2(R.s * xp.x + R.c * eq.x, R.s * xp.y + R.c * eq.y);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int e = tid; e < n * n; e += blockDim.x) {
+    const int i = e / n, j = e - i * n;
+    Vout[e] = sV[i][j];
+  }
+  for (int i = tid; i < n; i += blockDim.x) w[i] = sM[i][i].x;
+  if (tid == 0) *sweeps_out = sweeps;
+}
+
 __global__ void k_diag_real(const double2* __restrict__ M, int64_t n, double* __restrict__ w)
 {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -956,6 +1073,25 @@ int qsv_la_eigh(const qsv_mat* A, double* eigenvalues, qsv_mat* V, int* sweeps_o
   if (n > 16384) return api_fail(QSV_E_INVALID, "eigen-hermitian: matrix too large");
   if (V->data == A->data) return api_fail(QSV_E_INVALID, "eigen-hermitian: V aliases A");
   std::lock_guard<std::mutex> lk(g_la_mu);
+  if (n <= kJacSmall) {
+    double* w = nullptr;
+    if (hipMalloc(&w, (size_t)n * sizeof(double) + sizeof(int)) != hipSuccess)
+      return api_fail(QSV_E_NOMEM, "eigen-hermitian scratch allocation failed");
+    int* dsw = reinterpret_cast<int*>(w + n);
+    int sweeps = 0;
+    hipLaunchKernelGGL(k_jacobi_small, dim3(1), dim3(kLaBlock), 0, g_la_stream, (const double2*)A->data, V->data, w,
+                       (int)n, dsw);
+    const bool ok = hipGetLastError() == hipSuccess &&
+                    hipMemcpyAsync(eigenvalues, w, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, g_la_stream) ==
+                        hipSuccess &&
+                    hipMemcpyAsync(&sweeps, dsw, sizeof(int), hipMemcpyDeviceToHost, g_la_stream) == hipSuccess &&
+                    hipStreamSynchronize(g_la_stream) == hipSuccess;
+    hipFree(w);
+    if (!ok) return api_fail(QSV_E_DEVICE, "eigen-hermitian (single workgroup) failed");
+    if (sweeps >= 40) return api_fail(QSV_E_DEVICE, "eigen-hermitian: Jacobi did not converge");
+    if (sweeps_out) *sweeps_out = sweeps;
+    return QSV_OK;
+  }
   // round-robin schedule: N (even) players, N - 1 rounds of N / 2 disjoint pairs
   const int64_t N = n + (n & 1);
   const int npairs = (int)(N / 2), nrounds = (int)std::max<int64_t>(1, N - 1);
