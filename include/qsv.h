@@ -310,8 +310,9 @@ int qsv_mat_copy(const qsv_mat* src, qsv_mat* dst);        /* same element count
 /* shape (protocols.clj:106-114). */
 int qsv_mat_shape(const qsv_mat* m, int64_t* rows, int64_t* cols);
 /* A complex128 state's amplitudes as a 2^n x 1 matrix (no copy; the view must
- * not outlive the state). No counterpart in the reference: it lets results of
- * the engine enter the backend without a host round trip. */
+ * not outlive the state). Every call that takes the view first waits for the
+ * work queued on the state. No counterpart in the reference: it lets results
+ * of the engine enter the backend without a host round trip. */
 int qsv_mat_view_state(qsv_state* s, qsv_mat** out);
 /* add, subtract, scale, negate (protocols.clj:116-160): C = alpha A + beta B
  * (B may be NULL: C = alpha A). */

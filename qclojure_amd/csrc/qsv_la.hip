@@ -36,6 +36,7 @@ struct qsv_mat {
   double2* data = nullptr;
   bool owns = true;
   int device = 0;
+  hipStream_t dep = nullptr;   // a viewed state's stream: finish its work before touching the data
 };
 
 namespace {
@@ -723,6 +724,9 @@ unsigned grid_for(uint64_t n)
 int check_mat(const qsv_mat* m, const char* what)
 {
   if (!m || !m->data) return api_fail(QSV_E_INVALID, (std::string("null matrix: ") + what).c_str());
+  // a view of an engine state: its own stream may still be working on it
+  if (m->dep && hipStreamSynchronize(m->dep) != hipSuccess)
+    return api_fail(QSV_E_DEVICE, "waiting for the viewed state's stream failed");
   return QSV_OK;
 }
 
@@ -831,6 +835,7 @@ int qsv_mat_view_state(qsv_state* s, qsv_mat** out)
   m->cols = 1;
   m->data = (double2*)amps;
   m->owns = false;
+  m->dep = st;
   *out = m;
   return QSV_OK;
 }
@@ -838,7 +843,7 @@ int qsv_mat_view_state(qsv_state* s, qsv_mat** out)
 int qsv_la_axpby(double ar, double ai, const qsv_mat* A, double br, double bi, const qsv_mat* B, qsv_mat* C)
 {
   int rc = check_mat(A, "A");
-  if (rc || (rc = check_mat(C, "C"))) return rc;
+  if (rc || (rc = check_mat(C, "C")) || (B && (rc = check_mat(B, "B")))) return rc;
   if (B && (B->rows != A->rows || B->cols != A->cols))
     return api_fail(QSV_E_INVALID, "axpby: A and B shapes differ");
   if (C->rows != A->rows || C->cols != A->cols) return api_fail(QSV_E_INVALID, "axpby: C shape differs");
@@ -1012,7 +1017,7 @@ int qsv_la_deviation(int kind, const qsv_mat* A, double* out)
 int qsv_la_solve(const qsv_mat* A, const qsv_mat* B, qsv_mat* X)
 {
   int rc = check_mat(A, "A");
-  if (rc || (rc = check_mat(X, "X"))) return rc;
+  if (rc || (rc = check_mat(X, "X")) || (B && (rc = check_mat(B, "B")))) return rc;
   const int64_t n = A->rows;
   if (A->cols != n) return api_fail(QSV_E_INVALID, B ? "Matrix must be square for solving"
                                                      : "Matrix must be square for inverse");

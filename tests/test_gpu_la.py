@@ -226,4 +226,14 @@ def test_degenerate_spectrum_and_state_view(hip):
         assert hip.norm2(view) == 1.0
         amps = A_(view)[:, 0]
         assert amps[5] == 1 and np.count_nonzero(amps) == 1
+        # work queued on the state after the view was taken is finished before the view is read
+        st.apply(qc.ops_from_tuples([(qc.OP["H"], q, 0, 0, 0, 0.0) for q in range(6)] * 5))
+        amps = A_(view)[:, 0]
+        exp = np.zeros(64, complex)
+        exp[5] = 1
+        H = np.array([[1, 1], [1, -1]]) / np.sqrt(2)
+        Hn = np.ones((1, 1))
+        for _ in range(6):
+            Hn = np.kron(Hn, H)
+        assert np.max(np.abs(amps - Hn @ exp)) < 1e-12
         view.close()
