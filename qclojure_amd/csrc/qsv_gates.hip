@@ -12,6 +12,8 @@
 // the identity), so HBM sees full-line reads and writes. Loads and stores are
 // non-temporal: each byte of the state is touched once per gate.
 // All arithmetic is FP64 (complex64 states round once per amplitude per gate).
+#include <algorithm>
+
 #include "qsv_internal.h"
 
 namespace qsv {
@@ -93,10 +95,11 @@ __device__ __forceinline__ T cmul1(double dr, double di, T a)
 // each thread slot holds the pair. TLOW=true: target bit is a lane bit.
 // ---------------------------------------------------------------------------
 template <typename R, int PPT, bool TLOW, bool XCD>
-__global__ __launch_bounds__(kBlock) void k_gate1(typename NV<R>::T* __restrict__ psi, GateArgs g, Mat2<double> M)
+__global__ __launch_bounds__(kBlock) void k_gate1(typename NV<R>::T* __restrict__ psi, GateArgs g, Mat2<double> M,
+                                                   uint64_t c0)
 {
   using T = typename NV<R>::T;
-  const uint64_t base = block_chunk<XCD>() * (kBlock * PPT) + threadIdx.x;
+  const uint64_t base = (c0 + block_chunk<XCD>()) * (kBlock * PPT) + threadIdx.x;
   const uint64_t count = g.ins.count;
   uint64_t id[PPT];
   if constexpr (!TLOW) {
@@ -157,10 +160,10 @@ __global__ __launch_bounds__(kBlock) void k_gate1(typename NV<R>::T* __restrict_
 // ---------------------------------------------------------------------------
 template <typename R, int EPT>
 __global__ __launch_bounds__(kBlock) void k_phase(typename NV<R>::T* __restrict__ psi, GateArgs g, Cplx<double> d0,
-                                                   Cplx<double> d1)
+                                                   Cplx<double> d1, uint64_t c0)
 {
   using T = typename NV<R>::T;
-  const uint64_t base = block_chunk<true>() * (kBlock * EPT) + threadIdx.x;
+  const uint64_t base = (c0 + block_chunk<true>()) * (kBlock * EPT) + threadIdx.x;
   const uint64_t count = g.ins.count;
   T x[EPT];
   uint64_t id[EPT];
@@ -189,10 +192,11 @@ __global__ __launch_bounds__(kBlock) void k_phase(typename NV<R>::T* __restrict_
 // controlled. MODE 0: both high; 1: hA high, lB low; 2: both low.
 // ---------------------------------------------------------------------------
 template <typename R, int PPT, int MODE, bool XCD>
-__global__ __launch_bounds__(kBlock) void k_swap2(typename NV<R>::T* __restrict__ psi, SwapArgs s, Cplx<double> ph)
+__global__ __launch_bounds__(kBlock) void k_swap2(typename NV<R>::T* __restrict__ psi, SwapArgs s, Cplx<double> ph,
+                                                   uint64_t c0)
 {
   using T = typename NV<R>::T;
-  const uint64_t base = block_chunk<XCD>() * (kBlock * PPT) + threadIdx.x;
+  const uint64_t base = (c0 + block_chunk<XCD>()) * (kBlock * PPT) + threadIdx.x;
   const uint64_t count = s.ins.count;
   uint64_t id[PPT];
   T a[PPT], b[PPT];
@@ -257,10 +261,10 @@ __global__ __launch_bounds__(kBlock) void k_swap2(typename NV<R>::T* __restrict_
 // Rydberg blockade: phase where exactly one of the selected bits is set.
 template <typename R, int EPT>
 __global__ __launch_bounds__(kBlock) void k_blockade(typename NV<R>::T* __restrict__ psi, uint64_t S, uint64_t bmask,
-                                                      Cplx<double> ph)
+                                                      Cplx<double> ph, uint64_t c0)
 {
   using T = typename NV<R>::T;
-  const uint64_t base = block_chunk<true>() * (kBlock * EPT) + threadIdx.x;
+  const uint64_t base = (c0 + block_chunk<true>()) * (kBlock * EPT) + threadIdx.x;
   T x[EPT];
 #pragma unroll
   for (int k = 0; k < EPT; ++k) {
@@ -279,10 +283,22 @@ __global__ __launch_bounds__(kBlock) void k_blockade(typename NV<R>::T* __restri
 }
 
 // ---------------------------------------------------------------------------
-static inline unsigned blocks_for(uint64_t count, int per_thread)
+static inline uint64_t blocks_for(uint64_t count, int per_thread)
 {
   const uint64_t per_block = (uint64_t)kBlock * per_thread;
-  return (unsigned)((count + per_block - 1) / per_block);
+  return (count + per_block - 1) / per_block;
+}
+
+// A launch may hold fewer than 2^32 work-items: states of 34 qubits and more
+// take several launches of at most kMaxGrid workgroups, each starting at chunk
+// c0 (a multiple of 8, so the XCD-contiguous order stays a bijection).
+constexpr uint64_t kMaxGrid = 1ull << 23;
+
+template <typename F>
+static inline void launch_chunks(uint64_t blocks, F&& launch)
+{
+  for (uint64_t c0 = 0; c0 < blocks; c0 += kMaxGrid)
+    launch((unsigned)std::min<uint64_t>(kMaxGrid, blocks - c0), c0);
 }
 
 void launch_gate1(int prec, void* psi, const GateArgs& g, const double m[8], hipStream_t st)
@@ -298,30 +314,35 @@ void launch_gate1(int prec, void* psi, const GateArgs& g, const double m[8], hip
     const uint64_t dist = g.tbit * (prec == QSV_C128 ? 16u : 8u);
     if (dist >= (4u << 20)) {
       constexpr int PPT = 4;
-      const unsigned grid = blocks_for(g.ins.count, PPT);
-      if (prec == QSV_C128)
-        hipLaunchKernelGGL((k_gate1<double, PPT, false, false>), dim3(grid), dim3(kBlock), 0, st,
-                           (NV<double>::T*)psi, g, M);
-      else
-        hipLaunchKernelGGL((k_gate1<float, PPT, false, false>), dim3(grid), dim3(kBlock), 0, st,
-                           (NV<float>::T*)psi, g, M);
+      launch_chunks(blocks_for(g.ins.count, PPT), [&](unsigned grid, uint64_t c0) {
+        if (prec == QSV_C128)
+          hipLaunchKernelGGL((k_gate1<double, PPT, false, false>), dim3(grid), dim3(kBlock), 0, st,
+                             (NV<double>::T*)psi, g, M, c0);
+        else
+          hipLaunchKernelGGL((k_gate1<float, PPT, false, false>), dim3(grid), dim3(kBlock), 0, st,
+                             (NV<float>::T*)psi, g, M, c0);
+      });
     } else {
       constexpr int PPT = 2;
-      const unsigned grid = blocks_for(g.ins.count, PPT);
-      if (prec == QSV_C128)
-        hipLaunchKernelGGL((k_gate1<double, PPT, false, true>), dim3(grid), dim3(kBlock), 0, st,
-                           (NV<double>::T*)psi, g, M);
-      else
-        hipLaunchKernelGGL((k_gate1<float, PPT, false, true>), dim3(grid), dim3(kBlock), 0, st,
-                           (NV<float>::T*)psi, g, M);
+      launch_chunks(blocks_for(g.ins.count, PPT), [&](unsigned grid, uint64_t c0) {
+        if (prec == QSV_C128)
+          hipLaunchKernelGGL((k_gate1<double, PPT, false, true>), dim3(grid), dim3(kBlock), 0, st,
+                             (NV<double>::T*)psi, g, M, c0);
+        else
+          hipLaunchKernelGGL((k_gate1<float, PPT, false, true>), dim3(grid), dim3(kBlock), 0, st,
+                             (NV<float>::T*)psi, g, M, c0);
+      });
     }
   } else {
     constexpr int PPT = 4;
-    const unsigned grid = blocks_for(g.ins.count, PPT);
-    if (prec == QSV_C128)
-      hipLaunchKernelGGL((k_gate1<double, PPT, true, true>), dim3(grid), dim3(kBlock), 0, st, (NV<double>::T*)psi, g, M);
-    else
-      hipLaunchKernelGGL((k_gate1<float, PPT, true, true>), dim3(grid), dim3(kBlock), 0, st, (NV<float>::T*)psi, g, M);
+    launch_chunks(blocks_for(g.ins.count, PPT), [&](unsigned grid, uint64_t c0) {
+      if (prec == QSV_C128)
+        hipLaunchKernelGGL((k_gate1<double, PPT, true, true>), dim3(grid), dim3(kBlock), 0, st, (NV<double>::T*)psi,
+                           g, M, c0);
+      else
+        hipLaunchKernelGGL((k_gate1<float, PPT, true, true>), dim3(grid), dim3(kBlock), 0, st, (NV<float>::T*)psi,
+                           g, M, c0);
+    });
   }
 }
 
@@ -329,12 +350,13 @@ void launch_phase(int prec, void* psi, const GateArgs& g, const double d0[2], co
 {
   if (g.ins.count == 0) return;
   constexpr int EPT = 4;
-  const unsigned grid = blocks_for(g.ins.count, EPT);
-  const Cplx<double> c0{d0[0], d0[1]}, c1{d1[0], d1[1]};
-  if (prec == QSV_C128)
-    hipLaunchKernelGGL((k_phase<double, EPT>), dim3(grid), dim3(kBlock), 0, st, (NV<double>::T*)psi, g, c0, c1);
-  else
-    hipLaunchKernelGGL((k_phase<float, EPT>), dim3(grid), dim3(kBlock), 0, st, (NV<float>::T*)psi, g, c0, c1);
+  const Cplx<double> e0{d0[0], d0[1]}, e1{d1[0], d1[1]};
+  launch_chunks(blocks_for(g.ins.count, EPT), [&](unsigned grid, uint64_t c0) {
+    if (prec == QSV_C128)
+      hipLaunchKernelGGL((k_phase<double, EPT>), dim3(grid), dim3(kBlock), 0, st, (NV<double>::T*)psi, g, e0, e1, c0);
+    else
+      hipLaunchKernelGGL((k_phase<float, EPT>), dim3(grid), dim3(kBlock), 0, st, (NV<float>::T*)psi, g, e0, e1, c0);
+  });
 }
 
 void launch_swap2(int prec, void* psi, const SwapArgs& s, const double ph[2], hipStream_t st)
@@ -349,15 +371,14 @@ void launch_swap2(int prec, void* psi, const SwapArgs& s, const double ph[2], hi
   const uint64_t dist = mode == 0 ? (s.hA > s.hB ? s.hA - s.hB : s.hB - s.hA) : s.hA;
   const bool far = mode != 2 && dist * B >= (4ull << 20);
 #define QSV_SWAP_LAUNCH(MODE, PPT, XCD)                                                                         \
-  do {                                                                                                          \
-    const unsigned grid = blocks_for(s.ins.count, PPT);                                                         \
+  launch_chunks(blocks_for(s.ins.count, PPT), [&](unsigned grid, uint64_t c0) {                                 \
     if (prec == QSV_C128)                                                                                       \
       hipLaunchKernelGGL((k_swap2<double, PPT, MODE, XCD>), dim3(grid), dim3(kBlock), 0, st, (NV<double>::T*)psi, \
-                         s, p);                                                                                 \
+                         s, p, c0);                                                                             \
     else                                                                                                        \
       hipLaunchKernelGGL((k_swap2<float, PPT, MODE, XCD>), dim3(grid), dim3(kBlock), 0, st, (NV<float>::T*)psi,   \
-                         s, p);                                                                                 \
-  } while (0)
+                         s, p, c0);                                                                             \
+  })
   if (mode == 0) {
     if (far) QSV_SWAP_LAUNCH(0, 4, false);
     else QSV_SWAP_LAUNCH(0, 2, true);
@@ -373,12 +394,15 @@ void launch_swap2(int prec, void* psi, const SwapArgs& s, const double ph[2], hi
 void launch_blockade(int prec, void* psi, uint64_t S, uint64_t bmask, const double ph[2], hipStream_t st)
 {
   constexpr int EPT = 4;
-  const unsigned grid = blocks_for(S, EPT);
   const Cplx<double> p{ph[0], ph[1]};
-  if (prec == QSV_C128)
-    hipLaunchKernelGGL((k_blockade<double, EPT>), dim3(grid), dim3(kBlock), 0, st, (NV<double>::T*)psi, S, bmask, p);
-  else
-    hipLaunchKernelGGL((k_blockade<float, EPT>), dim3(grid), dim3(kBlock), 0, st, (NV<float>::T*)psi, S, bmask, p);
+  launch_chunks(blocks_for(S, EPT), [&](unsigned grid, uint64_t c0) {
+    if (prec == QSV_C128)
+      hipLaunchKernelGGL((k_blockade<double, EPT>), dim3(grid), dim3(kBlock), 0, st, (NV<double>::T*)psi, S, bmask, p,
+                         c0);
+    else
+      hipLaunchKernelGGL((k_blockade<float, EPT>), dim3(grid), dim3(kBlock), 0, st, (NV<float>::T*)psi, S, bmask, p,
+                         c0);
+  });
 }
 
 }  // namespace qsv

@@ -245,6 +245,31 @@ def test_large_state_round_trip(eng):
         assert abs(s.norm2() - 1.0) < 1e-12
 
 
+def test_largest_state_fits_and_computes(eng):
+    """The largest complex128 state HBM holds (34 qubits, 256 GiB): a GHZ
+    preparation across the outermost qubits plus a phase, checked on the
+    amplitudes it leaves nonzero and on the norm. Skipped when the free HBM
+    cannot hold it (other tests' cached buffers)."""
+    n = 34
+    need = 16 << n
+    free = eng.device_info()["hbm-free-bytes"]
+    if free < need + (1 << 28):
+        pytest.skip(f"needs {need / 2**30:.0f} GiB of free HBM, {free / 2**30:.0f} GiB free")
+    note("largest state: 34 qubits complex128")
+    ops = qc.ops_from_tuples([(OP["H"], 0, 0, 0, 0, 0.0), (OP["CNOT"], 0, n - 1, 0, 0, 0.0),
+                              (OP["RZ"], n // 2, 0, 0, 0, 0.5)])
+    with eng.DeviceState(n) as s:
+        s.apply(ops)
+        assert abs(s.norm2() - 1.0) < 1e-12
+        # qubit 0 is index bit n-1, qubit n-1 is bit 0: the two branches are
+        # index 0 and index 2^(n-1) + 1
+        lo = s.download(0, 4)
+        hi = s.download(1 << (n - 1), 4)
+        ph = np.exp(-0.25j) / np.sqrt(2)        # RZ(0.5) on a |0> qubit: e^{-i/4}
+        assert abs(lo[0] - ph) < 1e-12 and np.max(np.abs(lo[1:])) == 0
+        assert abs(hi[1] - ph) < 1e-12 and hi[0] == 0 and np.max(np.abs(hi[2:])) == 0
+
+
 def test_qaoa_p1_closed_form_on_device(eng):
     """Triangle-free 3-regular graphs, p = 1: <H_C> = m (1/2 + 1/2 sin4b sin g cos^2 g)."""
     from qclojure_amd.hamiltonian import hamiltonian_expectation
