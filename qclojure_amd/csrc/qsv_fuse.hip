@@ -209,8 +209,6 @@ __global__ __launch_bounds__(1 << (TB - kRegBits), (sizeof(R) == 4 && sizeof(A) 
   // time: 32 KiB instead of 64, so three workgroups fit a CU
   constexpr bool SPLIT = sizeof(T) == 16;
   __shared__ __align__(16) char smem[SPLIT ? (sizeof(R) << TB) : (sizeof(T) << TB)];
-  T* const lds = reinterpret_cast<T*>(smem);
-  R* const ldp = reinterpret_cast<R*>(smem);
   __shared__ double red[(1 << (TB - kRegBits)) / 64];
   if constexpr (GROUP) {
     const int64_t off = offs[blockIdx.y];
@@ -264,31 +262,53 @@ __global__ __launch_bounds__(1 << (TB - kRegBits), (sizeof(R) == 4 && sizeof(A) 
   };
 
   T x[kRegs];
-  // move the tile from layout `from` to layout `to` through LDS
-  auto relayout = [&](uint32_t from, uint32_t to) {
+  // LDS byte offset of slot k. swz is XOR-linear and a local index is the
+  // XOR of its thread part and its register part (disjoint bits), so the
+  // offset is swz(thread part) * E (one value per thread) XOR
+  // swz(register part of k) * E (workgroup-uniform, scalar arithmetic):
+  // one vector instruction per slot.
+  constexpr uint32_t E = SPLIT ? sizeof(R) : sizeof(T);
+  // (q0..q3: swz(r0..r3) * E; the register part of slot k is the XOR of
+  // those of its set bits, a few scalar XORs shared by all slots)
+  auto roff = [](int k, uint32_t q0, uint32_t q1, uint32_t q2, uint32_t q3) -> uint32_t {
+    return ((k & 1) ? q0 : 0u) ^ ((k & 2) ? q1 : 0u) ^ ((k & 4) ? q2 : 0u) ^ ((k & 8) ? q3 : 0u);
+  };
+  auto rsw = [](uint32_t r) -> uint32_t { return swz(r) * E; };
+  auto pl_st = [&](uint32_t off, R v) { *reinterpret_cast<R*>(smem + off) = v; };
+  auto pl_ld = [&](uint32_t off) -> R { return *reinterpret_cast<const R*>(smem + off); };
+  // move the tile from the current layout to layout `to` through LDS
+  // The per-thread part is made opaque so that the compiler keeps the
+  // scaling by E on the scalar side instead of factoring it out of the XOR.
+  auto tbase = [&]() -> uint32_t {
+    uint32_t b = swz(tl) * E;
+    asm("" : "+v"(b));
+    return b;
+  };
+  auto relayout = [&](uint32_t to) {
+    const uint32_t fb = tbase();
+    const uint32_t f0 = rsw(r0), f1 = rsw(r1), f2 = rsw(r2), f3 = rsw(r3);
+    set_layout(to);
+    const uint32_t tb = tbase();
+    const uint32_t t0 = rsw(r0), t1 = rsw(r1), t2 = rsw(r2), t3 = rsw(r3);
     if constexpr (SPLIT) {
 #pragma unroll
-      for (int k = 0; k < kRegs; ++k) ldp[swz(kl(k))] = x[k].x;
+      for (int k = 0; k < kRegs; ++k) pl_st(fb ^ roff(k, f0, f1, f2, f3), x[k].x);
       __syncthreads();
-      set_layout(to);
 #pragma unroll
-      for (int k = 0; k < kRegs; ++k) x[k].x = ldp[swz(kl(k))];
+      for (int k = 0; k < kRegs; ++k) x[k].x = pl_ld(tb ^ roff(k, t0, t1, t2, t3));
       __syncthreads();
-      set_layout(from);
 #pragma unroll
-      for (int k = 0; k < kRegs; ++k) ldp[swz(kl(k))] = x[k].y;
+      for (int k = 0; k < kRegs; ++k) pl_st(fb ^ roff(k, f0, f1, f2, f3), x[k].y);
       __syncthreads();
-      set_layout(to);
 #pragma unroll
-      for (int k = 0; k < kRegs; ++k) x[k].y = ldp[swz(kl(k))];
+      for (int k = 0; k < kRegs; ++k) x[k].y = pl_ld(tb ^ roff(k, t0, t1, t2, t3));
       __syncthreads();
     } else {
 #pragma unroll
-      for (int k = 0; k < kRegs; ++k) lds[swz(kl(k))] = x[k];
+      for (int k = 0; k < kRegs; ++k) *reinterpret_cast<T*>(smem + (fb ^ roff(k, f0, f1, f2, f3))) = x[k];
       __syncthreads();
-      set_layout(to);
 #pragma unroll
-      for (int k = 0; k < kRegs; ++k) x[k] = lds[swz(kl(k))];
+      for (int k = 0; k < kRegs; ++k) x[k] = *reinterpret_cast<const T*>(smem + (tb ^ roff(k, t0, t1, t2, t3)));
       __syncthreads();
     }
   };
@@ -319,7 +339,7 @@ __global__ __launch_bounds__(1 << (TB - kRegBits), (sizeof(R) == 4 && sizeof(A) 
   for (int s = 0; s < nst; ++s) {
     const FStage st = pass->stages[s];
     if (st.regmask != reg) {
-      relayout(reg, st.regmask);
+      relayout(st.regmask);
       reg = st.regmask;
     }
     for (int gi = st.g0; gi < st.g1; ++gi) {
@@ -373,39 +393,42 @@ __global__ __launch_bounds__(1 << (TB - kRegBits), (sizeof(R) == 4 && sizeof(A) 
         // every thread writes its slots to permuted LDS addresses
         const uint32_t ba = 1u << g->a, bb = 1u << g->b;
         const A pr = (A)g->m[0], pi = (A)g->m[1];
+        // a moving slot is stored at its partner's offset: swz(l ^ ba ^ bb)
+        // = swz(l) ^ swz(ba ^ bb)
+        const uint32_t sb = tbase(), dsw = swz(ba ^ bb) * E;
+        const uint32_t q0 = rsw(r0), q1 = rsw(r1), q2 = rsw(r2), q3 = rsw(r3);
+        auto moves = [&](int k) -> bool {
+          const uint32_t l = kl(k);
+          return ((l & cm) == cm) && (((l & ba) != 0) != ((l & bb) != 0));
+        };
         if constexpr (SPLIT) {
-          // real plane, then imaginary plane; each slot goes to its partner's address
+          // real plane, then imaginary plane
 #pragma unroll
           for (int k = 0; k < kRegs; ++k) {
-            const uint32_t l = kl(k);
-            const bool mv = ((l & cm) == cm) && (((l & ba) != 0) != ((l & bb) != 0));
+            const bool mv = moves(k);
             if (mv) x[k] = cscale<T, R, A>(pr, pi, x[k]);
-            ldp[swz(mv ? (l ^ ba ^ bb) : l)] = x[k].x;
+            pl_st(sb ^ roff(k, q0, q1, q2, q3) ^ (mv ? dsw : 0u), x[k].x);
           }
           __syncthreads();
 #pragma unroll
-          for (int k = 0; k < kRegs; ++k) x[k].x = ldp[swz(kl(k))];
+          for (int k = 0; k < kRegs; ++k) x[k].x = pl_ld(sb ^ roff(k, q0, q1, q2, q3));
           __syncthreads();
 #pragma unroll
-          for (int k = 0; k < kRegs; ++k) {
-            const uint32_t l = kl(k);
-            const bool mv = ((l & cm) == cm) && (((l & ba) != 0) != ((l & bb) != 0));
-            ldp[swz(mv ? (l ^ ba ^ bb) : l)] = x[k].y;
-          }
+          for (int k = 0; k < kRegs; ++k) pl_st(sb ^ roff(k, q0, q1, q2, q3) ^ (moves(k) ? dsw : 0u), x[k].y);
           __syncthreads();
 #pragma unroll
-          for (int k = 0; k < kRegs; ++k) x[k].y = ldp[swz(kl(k))];
+          for (int k = 0; k < kRegs; ++k) x[k].y = pl_ld(sb ^ roff(k, q0, q1, q2, q3));
           __syncthreads();
         } else {
 #pragma unroll
           for (int k = 0; k < kRegs; ++k) {
-            const uint32_t l = kl(k);
-            const bool mv = ((l & cm) == cm) && (((l & ba) != 0) != ((l & bb) != 0));
-            lds[swz(mv ? (l ^ ba ^ bb) : l)] = mv ? cscale<T, R, A>(pr, pi, x[k]) : x[k];
+            const bool mv = moves(k);
+            *reinterpret_cast<T*>(smem + (sb ^ roff(k, q0, q1, q2, q3) ^ (mv ? dsw : 0u))) =
+                mv ? cscale<T, R, A>(pr, pi, x[k]) : x[k];
           }
           __syncthreads();
 #pragma unroll
-          for (int k = 0; k < kRegs; ++k) x[k] = lds[swz(kl(k))];
+          for (int k = 0; k < kRegs; ++k) x[k] = *reinterpret_cast<const T*>(smem + (sb ^ roff(k, q0, q1, q2, q3)));
           __syncthreads();
         }
       } else {  // FG_BLOCKADE: phase where exactly one bit of bmask is set
@@ -419,7 +442,7 @@ __global__ __launch_bounds__(1 << (TB - kRegBits), (sizeof(R) == 4 && sizeof(A) 
   }
 
   if (reg != pass->store_mask) {
-    relayout(reg, pass->store_mask);
+    relayout(pass->store_mask);
     reg = pass->store_mask;
   }
   {
