@@ -137,6 +137,23 @@ __device__ __forceinline__ void reg_dense_dm(T (&x)[kRegs], const M8<A>& m, uint
   else reg_dense<RBIT, DM_GENERAL, ALL, T, R, A>(x, m, kc);
 }
 
+// Exchange of the slot pairs on register bit RBIT whose register-bit
+// controls kc are set, in the lanes where `act` holds (thread-bit controls):
+// selects only, so no lane-divergent branch.
+template <int RBIT, typename T>
+__device__ __forceinline__ void reg_xchg(T (&x)[kRegs], uint32_t kc, bool act)
+{
+#pragma unroll
+  for (int k = 0; k < kRegs; ++k) {
+    if (!(k & (1 << RBIT))) {
+      const bool sw = act && ((k & kc) == kc);
+      const T a = x[k], b = x[k | (1 << RBIT)];
+      x[k] = sw ? b : a;
+      x[k | (1 << RBIT)] = sw ? a : b;
+    }
+  }
+}
+
 // Diagonal diag(d0, d1) on register bit RBIT (skip0: d0 == 1); slot k acts
 // when (k & kc) == kc, every slot when ALL.
 template <int RBIT, bool ALL, typename T, typename R, typename A>
@@ -348,29 +365,58 @@ __global__ __launch_bounds__(1 << (TB - kRegBits), (sizeof(R) == 4 && sizeof(A) 
       const uint32_t cm = g->cmask;
       if (kind == FG_DENSE) {
         // register-resident controls become a slot mask, the others a
-        // per-thread predicate
+        // per-thread predicate. Uncontrolled gates take a path with no
+        // per-thread branch, so the slots are updated in place (a lane-
+        // divergent skip makes the compiler copy all 16 slots first).
         const uint32_t cthr = cm & ~reg;
-        if ((tl & cthr) != cthr) continue;
         const uint32_t kc = ((cm & r0) ? 1u : 0u) | ((cm & r1) ? 2u : 0u) | ((cm & r2) ? 4u : 0u) |
                             ((cm & r3) ? 8u : 0u);
         const M8<A> m{(A)g->m[0], (A)g->m[1], (A)g->m[2], (A)g->m[3],
                       (A)g->m[4], (A)g->m[5], (A)g->m[6], (A)g->m[7]};
         const uint32_t tb = 1u << g->a;
         const int dm = g->sub;
-        if (tb == r0) reg_dense_any<0, T, R, A>(x, m, kc, dm);
-        else if (tb == r1) reg_dense_any<1, T, R, A>(x, m, kc, dm);
-        else if (tb == r2) reg_dense_any<2, T, R, A>(x, m, kc, dm);
-        else reg_dense_any<3, T, R, A>(x, m, kc, dm);
+        if (cm == 0) {
+          if (tb == r0) reg_dense_dm<0, true, T, R, A>(x, m, 0u, dm);
+          else if (tb == r1) reg_dense_dm<1, true, T, R, A>(x, m, 0u, dm);
+          else if (tb == r2) reg_dense_dm<2, true, T, R, A>(x, m, 0u, dm);
+          else reg_dense_dm<3, true, T, R, A>(x, m, 0u, dm);
+        } else if (dm == DM_EXCHANGE) {
+          // controlled exchange (CNOT, Toffoli): per-lane selects, no branch
+          const bool act = (tl & cthr) == cthr;
+          if (tb == r0) reg_xchg<0, T>(x, kc, act);
+          else if (tb == r1) reg_xchg<1, T>(x, kc, act);
+          else if (tb == r2) reg_xchg<2, T>(x, kc, act);
+          else reg_xchg<3, T>(x, kc, act);
+        } else {
+          if ((tl & cthr) != cthr) continue;
+          if (tb == r0) reg_dense_any<0, T, R, A>(x, m, kc, dm);
+          else if (tb == r1) reg_dense_any<1, T, R, A>(x, m, kc, dm);
+          else if (tb == r2) reg_dense_any<2, T, R, A>(x, m, kc, dm);
+          else reg_dense_any<3, T, R, A>(x, m, kc, dm);
+        }
       } else if (kind == FG_PHASE) {
         // controls and target split like a dense gate's: thread-index bits
         // decide once per thread, register bits per slot at compile time
         const uint32_t tb = 1u << g->a;
         const uint32_t cthr = cm & ~reg;
-        if ((tl & cthr) != cthr) continue;
         const uint32_t kc = ((cm & r0) ? 1u : 0u) | ((cm & r1) ? 2u : 0u) | ((cm & r2) ? 4u : 0u) |
                             ((cm & r3) ? 8u : 0u);
         const A d0r = (A)g->m[0], d0i = (A)g->m[1], d1r = (A)g->m[6], d1i = (A)g->m[7];
         const bool skip0 = g->sub != 0;
+        if (cm == 0 && !skip0) {
+          // uncontrolled, both diagonal entries act: no per-thread branch
+          if (tb & reg) {
+            const int rb = (tb == r0) ? 0 : (tb == r1) ? 1 : (tb == r2) ? 2 : 3;
+            reg_phase_any<true, T, R, A>(x, rb, 0u, d0r, d0i, d1r, d1i, false);
+          } else {
+            const bool hi = (tl & tb) != 0;
+            const A dr = hi ? d1r : d0r, di = hi ? d1i : d0i;
+#pragma unroll
+            for (int k = 0; k < kRegs; ++k) x[k] = cscale<T, R, A>(dr, di, x[k]);
+          }
+          continue;
+        }
+        if ((tl & cthr) != cthr) continue;
         if (tb & reg) {
           const int rb = (tb == r0) ? 0 : (tb == r1) ? 1 : (tb == r2) ? 2 : 3;
           if (kc == 0) reg_phase_any<true, T, R, A>(x, rb, 0u, d0r, d0i, d1r, d1i, skip0);
