@@ -50,6 +50,7 @@ struct Pending {
 };
 
 void pool_release();
+void batch_cache_release();
 }  // namespace
 
 struct qsv_state {
@@ -1333,6 +1334,7 @@ void pool_release()
   if (g_group_arena) hipFree(g_group_arena);
   g_group_arena = nullptr;
   g_group_cap = 0;
+  batch_cache_release();
 }
 
 template <class T>
@@ -1343,6 +1345,58 @@ int dev_upload(const std::vector<T>& v, T** d, hipStream_t st)
   HIPCHK(hipMalloc((void**)d, v.size() * sizeof(T)));
   HIPCHK(hipMemcpyAsync(*d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
   return QSV_OK;
+}
+
+// Buffers and streams of qsv_batch_expect_z, kept from call to call (grow
+// only; used under the batch mutex, and only while the call's streams are
+// idle at its start): allocating and freeing them per call took a sizeable
+// part of a 4096-instance call.
+struct GrowBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  bool pinned = false;
+};
+struct BatchCache {
+  hipStream_t st = nullptr, gst = nullptr;
+  GrowBuf recs_h{nullptr, 0, true};   // pinned staging of the gate records
+  GrowBuf recs, insts, tz, tc, order, energy, bytes, offs;
+};
+BatchCache g_batch;
+
+int grow(GrowBuf& b, size_t bytes)
+{
+  bytes = std::max<size_t>(bytes, 256);
+  if (bytes <= b.cap) return QSV_OK;
+  if (b.p) HIPCHK(b.pinned ? hipHostFree(b.p) : hipFree(b.p));
+  b.p = nullptr;
+  b.cap = 0;
+  if (b.pinned) HIPCHK(hipHostMalloc(&b.p, bytes, hipHostMallocDefault));
+  else HIPCHK(hipMalloc(&b.p, bytes));
+  b.cap = bytes;
+  return QSV_OK;
+}
+
+// Copies n elements to a grow-only device buffer on stream st.
+template <class T>
+int upload(GrowBuf& b, const T* src, size_t n, hipStream_t st)
+{
+  int rc = grow(b, n * sizeof(T));
+  if (rc) return rc;
+  if (n) HIPCHK(hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, st));
+  return QSV_OK;
+}
+
+void batch_cache_release()
+{
+  for (GrowBuf* b : {&g_batch.recs_h, &g_batch.recs, &g_batch.insts, &g_batch.tz, &g_batch.tc, &g_batch.order,
+                     &g_batch.energy, &g_batch.bytes, &g_batch.offs}) {
+    if (b->p) (void)(b->pinned ? hipHostFree(b->p) : hipFree(b->p));
+    b->p = nullptr;
+    b->cap = 0;
+  }
+  if (g_batch.st) hipStreamDestroy(g_batch.st);
+  if (g_batch.gst) hipStreamDestroy(g_batch.gst);
+  g_batch.st = g_batch.gst = nullptr;
 }
 
 }  // namespace
@@ -1368,10 +1422,11 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
   const size_t B = prec == QSV_C128 ? 16 : 8;
   const int lds_n = prec == QSV_C128 ? 13 : 14;   // 2^n * B <= 128 KiB of LDS
 
-  std::vector<BRec> recs;
+  const uint64_t T0 = d->term_offsets[0];
+  const uint64_t n_terms = d->term_offsets[N] - T0;
   std::vector<BInst> insts(N);
-  std::vector<uint64_t> tz;
-  std::vector<double> tc;
+  std::vector<uint64_t> tz(n_terms);
+  std::vector<double> tc(n_terms);
   std::vector<std::vector<BGate>> big_gates;
   std::vector<uint64_t> big_idx;
   std::vector<int32_t> small_idx;
@@ -1380,10 +1435,12 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
     const int n = d->num_qubits[i];
     if (n < 1 || n > 34) return fail(QSV_E_INVALID, "instance " + std::to_string(i) + ": num_qubits out of range");
   }
-  // Decode every instance, then fuse the programs of the instances too large
-  // for LDS into grouped passes: independent per instance, so both run on a
-  // pool of host threads (at most 16); errors are reported for the first
-  // failing instance in order, as a sequential decode would.
+  // Decode every instance, convert its Z terms, and fuse the programs of the
+  // instances too large for LDS into grouped passes; then write the small
+  // instances' gate records into pinned staging memory. Both phases are
+  // independent per instance and run on a pool of host threads (at most 16);
+  // errors are reported for the first failing instance in order, decode errors
+  // before term errors, as a sequential pass would.
   std::vector<std::vector<BGate>> decoded(N);
   struct Plan {
     std::vector<char> bytes;
@@ -1393,75 +1450,89 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
   std::vector<Plan> plans(N);
   std::vector<int> dec_rc(N, 0);
   std::vector<std::string> dec_err(N);
-  {
-    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    const unsigned nt = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(hw, N / 8 + 1));
+  std::vector<char> term_bad(N, 0);
+  std::vector<uint64_t> rec_off(N + 1, 0);
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const unsigned nt = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(hw, N / 8 + 1));
+  auto parallel = [&](auto&& body) {
     std::atomic<uint64_t> next{0};
     auto worker = [&]() {
-      for (uint64_t i; (i = next.fetch_add(1)) < N;) {
-        const int n = d->num_qubits[i];
-        uint64_t mk = 0;
-        for (uint64_t k = d->op_offsets[i]; k < d->op_offsets[i + 1]; ++k) {
-          if (d->ops[k].kind == QSV_OP_MEASURE) {
-            dec_rc[i] = QSV_E_UNSUPPORTED;
-            dec_err[i] = "instance " + std::to_string(i) + ": measurements are not batched";
-            break;
-          }
-          const int r = decode_op(n, d->ops[k], 0, 0, &mk, decoded[i]);
-          if (r) {
-            dec_rc[i] = r;
-            dec_err[i] = "instance " + std::to_string(i) + ", op " + std::to_string(k - d->op_offsets[i]) + ": " + g_err;
-            break;
-          }
-        }
-        if (!dec_rc[i] && n > lds_n) plans[i].ok = plan_group_passes(n, prec, decoded[i], plans[i].bytes, plans[i].offs);
-      }
+      for (uint64_t i; (i = next.fetch_add(1)) < N;) body(i);
     };
     std::vector<std::thread> pool;
     for (unsigned t = 1; t < nt; ++t) pool.emplace_back(worker);
     worker();
     for (auto& t : pool) t.join();
-  }
-  for (uint64_t i = 0; i < N; ++i)
-    if (dec_rc[i]) return fail(dec_rc[i], dec_err[i]);
-  std::vector<Plan> big_plans;
-  for (uint64_t i = 0; i < N; ++i) {
+  };
+  parallel([&](uint64_t i) {
     const int n = d->num_qubits[i];
-    std::vector<BGate>& gs = decoded[i];
-    BInst& in = insts[i];
-    in.n = n;
-    in.t0 = tz.size();
+    uint64_t mk = 0;
+    for (uint64_t k = d->op_offsets[i]; k < d->op_offsets[i + 1]; ++k) {
+      if (d->ops[k].kind == QSV_OP_MEASURE) {
+        dec_rc[i] = QSV_E_UNSUPPORTED;
+        dec_err[i] = "instance " + std::to_string(i) + ": measurements are not batched";
+        break;
+      }
+      const int r = decode_op(n, d->ops[k], 0, 0, &mk, decoded[i]);
+      if (r) {
+        dec_rc[i] = r;
+        dec_err[i] = "instance " + std::to_string(i) + ", op " + std::to_string(k - d->op_offsets[i]) + ": " + g_err;
+        break;
+      }
+    }
+    // Z masks with bit q = Z on qubit q, to basis-index bits (qubit q = bit n-1-q)
     for (uint64_t k = d->term_offsets[i]; k < d->term_offsets[i + 1]; ++k) {
       const uint64_t qm = d->term_zmask[k];
-      if (n < 64 && (qm >> n)) return fail(QSV_E_INVALID, "instance " + std::to_string(i) + ": Z term on a missing qubit");
+      if (n < 64 && (qm >> n)) term_bad[i] = 1;
       uint64_t bm = 0;
       for (int q = 0; q < n; ++q)
         if ((qm >> q) & 1ull) bm |= 1ull << (n - 1 - q);
-      tz.push_back(bm);
-      tc.push_back(d->term_coeff[k]);
+      tz[k - T0] = bm;
+      tc[k - T0] = d->term_coeff[k];
     }
-    in.t1 = tz.size();
+    if (!dec_rc[i] && n > lds_n) plans[i].ok = plan_group_passes(n, prec, decoded[i], plans[i].bytes, plans[i].offs);
+  });
+  for (uint64_t i = 0; i < N; ++i)
+    if (dec_rc[i]) return fail(dec_rc[i], dec_err[i]);
+  for (uint64_t i = 0; i < N; ++i)
+    if (term_bad[i]) return fail(QSV_E_INVALID, "instance " + std::to_string(i) + ": Z term on a missing qubit");
+  for (uint64_t i = 0; i < N; ++i)
+    rec_off[i + 1] = rec_off[i] + (d->num_qubits[i] <= lds_n ? decoded[i].size() : 0);
+  const uint64_t n_recs = rec_off[N];
+  if ((rc = grow(g_batch.recs_h, n_recs * sizeof(BRec)))) return rc;
+  BRec* const recs = static_cast<BRec*>(g_batch.recs_h.p);
+  parallel([&](uint64_t i) {
+    if (d->num_qubits[i] > lds_n) return;
+    BRec* out = recs + rec_off[i];
+    for (const BGate& g : decoded[i]) {
+      BRec r{};
+      r.kind = g.kind;
+      r.a = g.a;
+      r.b = g.b;
+      r.phase_only = g.phase_only ? 1 : 0;
+      for (int c : g.cbits) r.cmask |= 1ull << c;
+      r.bmask = g.bmask;
+      std::memcpy(r.m, g.m, sizeof(r.m));
+      *out++ = r;
+    }
+  });
+  std::vector<Plan> big_plans;
+  for (uint64_t i = 0; i < N; ++i) {
+    const int n = d->num_qubits[i];
+    BInst& in = insts[i];
+    in.n = n;
+    in.t0 = d->term_offsets[i] - T0;
+    in.t1 = d->term_offsets[i + 1] - T0;
     in.out_offset = ~0ull;
     if (n <= lds_n) {
-      in.g0 = recs.size();
-      for (const BGate& g : gs) {
-        BRec r{};
-        r.kind = g.kind;
-        r.a = g.a;
-        r.b = g.b;
-        r.phase_only = g.phase_only ? 1 : 0;
-        for (int c : g.cbits) r.cmask |= 1ull << c;
-        r.bmask = g.bmask;
-        std::memcpy(r.m, g.m, sizeof(r.m));
-        recs.push_back(r);
-      }
-      in.g1 = recs.size();
+      in.g0 = rec_off[i];
+      in.g1 = rec_off[i + 1];
       small_idx.push_back((int32_t)i);
       max_small_n = std::max(max_small_n, n);
     } else {
       in.g0 = in.g1 = 0;
       big_idx.push_back(i);
-      big_gates.push_back(std::move(gs));
+      big_gates.push_back(std::move(decoded[i]));
       big_plans.push_back(std::move(plans[i]));
     }
   }
@@ -1469,36 +1540,29 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
   std::stable_sort(small_idx.begin(), small_idx.end(),
                    [&](int32_t a, int32_t b) { return insts[a].n > insts[b].n; });
 
-  hipStream_t st = nullptr;
-  HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-  BRec* d_recs = nullptr;
-  BInst* d_insts = nullptr;
-  uint64_t* d_tz = nullptr;
-  double* d_tc = nullptr;
-  int32_t* d_order = nullptr;
-  double* d_energy = nullptr;
+  if (!g_batch.st && hipStreamCreateWithFlags(&g_batch.st, hipStreamNonBlocking) != hipSuccess)
+    return fail(QSV_E_DEVICE, "batch: stream creation failed");
+  hipStream_t st = g_batch.st;
+  hipEvent_t uploaded = nullptr;
   auto cleanup = [&]() {
     hipStreamSynchronize(st);
-    hipFree(d_recs);
-    hipFree(d_insts);
-    hipFree(d_tz);
-    hipFree(d_tc);
-    hipFree(d_order);
-    hipFree(d_energy);
-    hipStreamDestroy(st);
+    if (uploaded) hipEventDestroy(uploaded);
+    uploaded = nullptr;
   };
-  if ((rc = dev_upload(recs, &d_recs, st)) || (rc = dev_upload(insts, &d_insts, st)) ||
-      (rc = dev_upload(tz, &d_tz, st)) || (rc = dev_upload(tc, &d_tc, st)) ||
-      (rc = dev_upload(small_idx, &d_order, st))) {
+  if ((rc = upload(g_batch.recs, recs, n_recs, st)) || (rc = upload(g_batch.insts, insts.data(), N, st)) ||
+      (rc = upload(g_batch.tz, tz.data(), tz.size(), st)) || (rc = upload(g_batch.tc, tc.data(), tc.size(), st)) ||
+      (rc = upload(g_batch.order, small_idx.data(), small_idx.size(), st)) ||
+      (rc = grow(g_batch.energy, N * sizeof(double)))) {
     cleanup();
     return rc;
   }
-  if (hipMalloc(&d_energy, N * sizeof(double)) != hipSuccess) {
-    cleanup();
-    return fail(QSV_E_NOMEM, "batch: energies allocation failed");
-  }
+  BRec* const d_recs = static_cast<BRec*>(g_batch.recs.p);
+  BInst* const d_insts = static_cast<BInst*>(g_batch.insts.p);
+  uint64_t* const d_tz = static_cast<uint64_t*>(g_batch.tz.p);
+  double* const d_tc = static_cast<double*>(g_batch.tc.p);
+  int32_t* const d_order = static_cast<int32_t*>(g_batch.order.p);
+  double* const d_energy = static_cast<double*>(g_batch.energy.p);
   // the pooled states' streams read the uploaded terms: order them after the copies
-  hipEvent_t uploaded = nullptr;
   if (hipEventCreateWithFlags(&uploaded, hipEventDisableTiming) != hipSuccess ||
       hipEventRecord(uploaded, st) != hipSuccess) {
     cleanup();
@@ -1523,22 +1587,17 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
     std::map<int, std::vector<size_t>, std::greater<int>> by_n;   // n (largest first) -> positions in big_idx
     for (size_t k = 0; k < big_idx.size(); ++k) by_n[insts[big_idx[k]].n].push_back(k);
     hipStream_t gst = nullptr;
-    char* d_bytes = nullptr;
-    int64_t* d_offs = nullptr;
     double* d_part = nullptr;
     uint64_t* d_meta = nullptr;
     size_t part_cap = 0, meta_cap = 0;
     auto gclean = [&]() {
       if (gst) hipStreamSynchronize(gst);
-      hipFree(d_bytes);
-      hipFree(d_offs);
       hipFree(d_part);
       hipFree(d_meta);
-      if (gst) hipStreamDestroy(gst);
     };
     if (!by_n.empty()) {
-      if (hipStreamCreateWithFlags(&gst, hipStreamNonBlocking) != hipSuccess ||
-          hipStreamWaitEvent(gst, uploaded, 0) != hipSuccess) {
+      if ((!g_batch.gst && hipStreamCreateWithFlags(&g_batch.gst, hipStreamNonBlocking) != hipSuccess) ||
+          hipStreamWaitEvent(gst = g_batch.gst, uploaded, 0) != hipSuccess) {
         gclean();
         cleanup();
         return fail(QSV_E_DEVICE, "batch: group stream setup failed");
@@ -1583,20 +1642,14 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
           cleanup();
           return rc;
         }
-        hipFree(d_bytes);
-        hipFree(d_offs);
-        d_bytes = nullptr;
-        d_offs = nullptr;
-        if (hipMalloc((void**)&d_bytes, std::max<size_t>(bytes.size(), 256)) != hipSuccess ||
-            hipMalloc((void**)&d_offs, table.size() * sizeof(int64_t)) != hipSuccess ||
-            (!bytes.empty() &&
-             hipMemcpyAsync(d_bytes, bytes.data(), bytes.size(), hipMemcpyHostToDevice, gst) != hipSuccess) ||
-            hipMemcpyAsync(d_offs, table.data(), table.size() * sizeof(int64_t), hipMemcpyHostToDevice, gst) !=
-                hipSuccess) {
+        if (upload(g_batch.bytes, bytes.data(), bytes.size(), gst) ||
+            upload(g_batch.offs, table.data(), table.size(), gst)) {
           gclean();
           cleanup();
           return fail(QSV_E_NOMEM, "batch: group descriptor upload failed");
         }
+        const char* const d_bytes = static_cast<const char*>(g_batch.bytes.p);
+        const int64_t* const d_offs = static_cast<const int64_t*>(g_batch.offs.p);
         launch_fill_group(prec, g_group_arena, S, c, gst);
         for (size_t r = 0; r < rounds; ++r)
           launch_tile_group(prec, g_group_arena, S, (unsigned)c, d_bytes, d_offs + r * c, gst);
@@ -1660,7 +1713,6 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
   if (e == hipSuccess) e = hipMemcpyAsync(energies, d_energy, N * sizeof(double), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   cleanup();
-  hipEventDestroy(uploaded);
   if (e != hipSuccess) return fail(QSV_E_DEVICE, std::string("batch: ") + hipGetErrorString(e));
   return QSV_OK;
 }
