@@ -595,13 +595,14 @@ __global__ void k_identity(double2* __restrict__ V, int64_t n)
 // The whole Jacobi iteration for n <= kJacSmall in one workgroup, A and V in
 // LDS: the same rotations, round-robin schedule and stopping rule as the
 // multi-launch path, with barriers in place of launches.
-constexpr int kJacSmall = 32;
+constexpr int kJacSmall = 64;
+constexpr int kJacPad = kJacSmall + 1;   // row stride: a column walk crosses LDS banks
 
 __global__ __launch_bounds__(256) void k_jacobi_small(const double2* __restrict__ A, double2* __restrict__ Vout,
                                                       double* __restrict__ w, int n, int* __restrict__ sweeps_out)
 {
-  __shared__ double2 sM[kJacSmall][kJacSmall];
-  __shared__ double2 sV[kJacSmall][kJacSmall];
+  __shared__ double2 sM[kJacSmall][kJacPad];
+  __shared__ double2 sV[kJacSmall][kJacPad];
   __shared__ Rot rot[kJacSmall / 2];
   __shared__ int2 pr[kJacSmall / 2];
   __shared__ double ra[kLaBlock / 64], rb[kLaBlock / 64];
