@@ -4,7 +4,10 @@ complex_linear_algebra.clj).
 
 ``HipBackend`` implements BackendAdapter (protocols.clj:12-76), MatrixAlgebra
 (protocols.clj:81-353) and eigen-hermitian (protocols.clj:363-373) with one
-libqsv call per operation; its matrices are opaque device handles
+libqsv call per operation, and on top of those device calls svd, the matrix
+functions (exp; log and sqrt of diagonal and Hermitian matrices) and
+MatrixAnalysis (protocols.clj:357-520); eigen-general, LU, QR and Cholesky
+raise LinearAlgebraError; its matrices are opaque device handles
 (``DeviceMatrix``), which the protocol allows ("implementations may return
 opaque handles"), so chained operations stay in HBM and only ``to_numpy()``
 reads them back. ``kronecker-product`` results are kept as their factors
@@ -451,6 +454,181 @@ class HipBackend:
                 "sweeps": sweeps.value}
 
 
+    # -- device helpers for the decompositions, functions and analysis below
+    def _gemm(self, A: DeviceMatrix, B: DeviceMatrix, conja: int = 0) -> DeviceMatrix:
+        C = DeviceMatrix(A.cols if conja else A.rows, B.cols)
+        _call(_lib().qsv_la_gemm, conja, A.handle, B.handle, C.handle)
+        return C
+
+    def _eigh_dev(self, A: DeviceMatrix):
+        """Eigenvalues (host, unsorted) and unit eigenvector columns (device)."""
+        n = A.rows
+        w = np.empty(n, dtype=np.float64)
+        V = DeviceMatrix(n, n)
+        sweeps = ctypes.c_int(0)
+        _call(_lib().qsv_la_eigh, A.handle, w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), V.handle,
+              ctypes.byref(sweeps))
+        return w, V
+
+    def _mul_cols(self, V: DeviceMatrix, d) -> DeviceMatrix:
+        """V · diag(d): the elementwise product with the rank-one matrix 1 dᵀ
+        (an outer product conjugates its second vector, so it gets conj(d))."""
+        ones = DeviceMatrix.from_numpy(np.ones((V.rows, 1), dtype=np.complex128))
+        dc = DeviceMatrix.from_numpy(np.conj(np.asarray(d, dtype=np.complex128)).reshape(-1, 1))
+        D = DeviceMatrix(V.rows, V.cols)
+        _call(_lib().qsv_la_outer, ones.handle, dc.handle, D.handle)
+        return self.hadamard_product(V, D)
+
+    def _spectral_fn(self, w, V: DeviceMatrix, f) -> DeviceMatrix:
+        """V f(Λ) Vᴴ for unit eigenvector columns V of a normal matrix."""
+        return self._gemm(self._mul_cols(V, [f(x) for x in w]), self.conjugate_transpose(V))
+
+    def _ata_eigen(self, A):
+        A = self._dense(A)
+        return self._eigh_dev(self._gemm(A, A, conja=1))
+
+    def _square(self, A, what: str) -> DeviceMatrix:
+        A = self._dense(A)
+        if A.rows != A.cols:
+            raise LinearAlgebraError(f"{what} requires square matrix", {"shape": A.shape})
+        return A
+
+    # -- MatrixAnalysis (protocols.clj:494-520)
+    def spectral_norm(self, A) -> float:
+        """spectral-norm (fastmath/complex_linear_algebra.clj:1409-1436): the
+        square root of the largest eigenvalue of AᴴA (clamped at 0, so that a
+        zero matrix gives 0)."""
+        w, _ = self._ata_eigen(A)
+        return float(np.sqrt(max(0.0, float(np.max(w)))))
+
+    def condition_number(self, A) -> float:
+        """condition-number (:1438-1470): σ_max / σ_min from the eigenvalues of
+        AᴴA, +inf when σ_min < 1e-14."""
+        w, _ = self._ata_eigen(A)
+        sv = np.sqrt(np.maximum(0.0, w))
+        smax, smin = float(np.max(sv)), float(np.min(sv))
+        return float("inf") if smin < 1e-14 else smax / smin
+
+    # -- MatrixDecompositions (protocols.clj:357-444)
+    def svd(self, A) -> dict:
+        """svd (:883-963) as the reference computes it: V and σ from the
+        eigendecomposition of AᴴA (σ = sqrt(max(0, λ)), descending, ties in
+        eigenvalue order), U's column j = A v_j / σ_j (zero when σ_j <= 1e-12)
+        for j < min(m, n), then for m > n the unit vectors e_n … e_{m-1}.
+        Returns {"U": m x m, "S": σ (n values, complex), "V†": Vᴴ}."""
+        A = self._dense(A)
+        m, n = A.rows, A.cols
+        w, V = self._ata_eigen(A)
+        sv = np.sqrt(np.maximum(0.0, w))
+        order = sorted(range(n), key=lambda j: -sv[j])            # stable: ties keep eigenvalue order
+        P = np.zeros((n, n), dtype=np.complex128)                  # V P = V with sorted columns
+        P[order, np.arange(n)] = 1.0
+        Vs = self._gemm(V, DeviceMatrix.from_numpy(P))
+        k = min(m, n)
+        PS = np.zeros((n, m), dtype=np.complex128)                 # column j < k of U scaled by 1/σ_j
+        for j in range(k):
+            s = sv[order[j]]
+            if s > 1e-12:
+                PS[j, j] = 1.0 / s
+        U = self._gemm(A, self._gemm(Vs, DeviceMatrix.from_numpy(PS)))
+        if m > n:
+            E = np.zeros((m, m), dtype=np.complex128)
+            E[np.arange(n, m), np.arange(n, m)] = 1.0
+            U = self.add(U, DeviceMatrix.from_numpy(E))
+        return {"U": U, "S": [complex(sv[j], 0.0) for j in order], "V†": self.conjugate_transpose(Vs)}
+
+    def _not_provided(self, name: str):
+        raise LinearAlgebraError(f"{name} is not provided by the :hip backend",
+                                 {"provided": ["eigen-hermitian", "svd"]})
+
+    def eigen_general(self, A):
+        self._not_provided("eigen-general")
+
+    def lu_decomposition(self, A):
+        self._not_provided("lu-decomposition")
+
+    def qr_decomposition(self, A):
+        self._not_provided("qr-decomposition")
+
+    def cholesky_decomposition(self, A, eps: float | None = None):
+        self._not_provided("cholesky-decomposition")
+
+    # -- MatrixFunctions (protocols.clj:446-492)
+    def matrix_exp(self, A):
+        """matrix-exp (:1168-1258). Hermitian A (tolerance 1e-10): V e^Λ Vᴴ;
+        skew-Hermitian A = iH: V e^{iΛ} Vᴴ from H's eigensystem; any other A:
+        scaling and squaring of the Taylor series, all on the device."""
+        A = self._square(A, "Matrix exponential")
+        n = A.rows
+        if self.hermitian(A, 1e-10):
+            w, V = self._eigh_dev(A)
+            return self._spectral_fn(w, V, lambda x: complex(np.exp(x)))
+        H = self.scale(A, -1j)
+        if self.hermitian(H, 1e-10):
+            w, V = self._eigh_dev(H)
+            return self._spectral_fn(w, V, lambda x: complex(np.cos(x), np.sin(x)))
+        nrm = self.norm2(A)
+        s = max(0, int(np.ceil(np.log2(nrm))) + 1) if nrm > 0.5 else 0
+        B = self.scale(A, 2.0 ** -s)
+        I = DeviceMatrix.from_numpy(np.eye(n, dtype=np.complex128))
+        T = self.add(I, B)
+        term = B
+        for k in range(2, 40):
+            term = self.scale(self._gemm(term, B), 1.0 / k)
+            T = self.add(T, term)
+            if self.norm2(term) <= 1e-17 * self.norm2(T):
+                break
+        for _ in range(s):
+            T = self._gemm(T, T)
+        return T
+
+    def _diag_fn(self, A: DeviceMatrix, f, what: str):
+        d = np.diag(A.to_numpy())
+        if np.any(np.abs(d) < 1e-14):
+            raise LinearAlgebraError(f"Cannot compute {what} of zero diagonal element")
+        return DeviceMatrix.from_numpy(np.diag([f(z) for z in d]).astype(np.complex128))
+
+    def _hermitian_fn(self, A: DeviceMatrix, f, what: str, failed: str):
+        if not self.hermitian(A, 1e-10):
+            raise LinearAlgebraError(failed, {"original-error": "a non-Hermitian, non-diagonal matrix needs "
+                                              "eigen-general, which the :hip backend does not provide",
+                                              "matrix-size": A.rows})
+        w, V = self._eigh_dev(A)
+        if np.any(np.abs(w) < 1e-14):
+            raise LinearAlgebraError(failed, {"original-error": f"Cannot compute {what} of zero eigenvalue",
+                                              "matrix-size": A.rows})
+        return self._spectral_fn(w, V, f)
+
+    @staticmethod
+    def _log(z) -> complex:
+        z = complex(z)
+        return complex(np.log(abs(z)), np.angle(z))
+
+    @staticmethod
+    def _sqrt(z) -> complex:
+        z = complex(z)
+        r, th = np.sqrt(abs(z)), np.angle(z) / 2.0
+        return complex(r * np.cos(th), r * np.sin(th))
+
+    def matrix_log(self, A):
+        """matrix-log (:1260-1329): principal branch, log|λ| + i arg λ; a
+        diagonal A (1e-10) entry by entry, a Hermitian A through its
+        eigensystem."""
+        A = self._square(A, "Matrix logarithm")
+        if self.diagonal(A, 1e-10):
+            return self._diag_fn(A, self._log, "log")
+        return self._hermitian_fn(A, self._log, "log", "Complex matrix logarithm failed")
+
+    def matrix_sqrt(self, A):
+        """matrix-sqrt (:1331-1407): principal branch, sqrt|λ| e^{i arg(λ)/2};
+        a diagonal A (1e-10) entry by entry, a Hermitian A through its
+        eigensystem."""
+        A = self._square(A, "Matrix square root")
+        if self.diagonal(A, 1e-10):
+            return self._diag_fn(A, self._sqrt, "sqrt")
+        return self._hermitian_fn(A, self._sqrt, "sqrt", "Complex matrix square root failed")
+
+
 # ---- facade (complex_linear_algebra.clj:18-740) ----------------------------------------------
 _BACKENDS = {"hip": HipBackend}
 _state = threading.local()
@@ -528,6 +706,15 @@ def _facade(name):
 for _n in ("shape", "add", "subtract", "scale", "negate", "matrix_multiply", "matrix_vector_product",
            "inner_product", "outer_product", "hadamard_product", "kronecker_product", "transpose",
            "conjugate_transpose", "trace", "norm2", "solve_linear_system", "inverse", "hermitian", "diagonal",
-           "unitary", "positive_semidefinite", "eigen_hermitian"):
+           "unitary", "positive_semidefinite", "eigen_hermitian", "eigen_general", "lu_decomposition",
+           "qr_decomposition", "cholesky_decomposition", "matrix_exp", "matrix_log", "matrix_sqrt",
+           "spectral_norm", "condition_number"):
     globals()[_n] = _facade(_n)
 del _n
+
+
+def svd(A) -> dict:
+    """svd through the facade (complex_linear_algebra.clj:568-588): the
+    backend's {:U :S :V†} renamed to {"U", "singular-values", "Vt"}."""
+    r = get_backend_instance().svd(A)
+    return {"U": r["U"], "singular-values": r["S"], "Vt": r["V†"]}

@@ -93,3 +93,77 @@ def test_predicates_and_psd():
     assert not lo.positive_semidefinite(np.array([[1, 2], [2, 1]], complex), 1e-12)
     with pytest.raises(lo.OracleLAError):
         lo.positive_semidefinite(np.array([[1, 2], [3, 1]], complex), 1e-12)
+
+
+# ---- MatrixAnalysis, svd, matrix functions: the expectations of the reference's
+# complex_linear_algebra_test.clj:156-326, then scipy / numpy on other inputs
+
+def test_reference_function_expectations():
+    assert np.allclose(lo.matrix_exp(np.zeros((2, 2))), np.eye(2), atol=1e-10)
+    E = lo.matrix_exp(np.diag([1.0, 2.0]))
+    assert abs(E[0, 0] - np.exp(1)) < 1e-10 and abs(E[1, 1] - np.exp(2)) < 1e-10 and abs(E[0, 1]) < 1e-10
+    assert abs(lo.matrix_exp([[3.0]])[0, 0] - np.exp(3)) < 1e-10
+    U = lo.matrix_exp(np.diag([-1j * np.pi / 4, 1j * np.pi / 4]))
+    assert lo.unitary(U, 1e-12)
+    c = np.sqrt(2) / 2
+    assert abs(U[0, 0] - complex(c, -c)) < 1e-10 and abs(U[1, 1] - complex(c, c)) < 1e-10
+    assert lo.unitary(lo.matrix_exp(np.diag([-1j, 1j])), 1e-12)
+    assert np.allclose(lo.matrix_log(np.eye(2)), 0, atol=1e-12)
+    L = lo.matrix_log(np.diag([2.0, 3.0]))
+    assert abs(L[0, 0] - np.log(2)) < 1e-10 and abs(L[1, 1] - np.log(3)) < 1e-10 and abs(L[0, 1]) < 1e-12
+    assert np.max(np.abs(lo.matrix_log(np.array([[1.0, 2.0], [2.0, -1.0]])).imag)) > 1e-10
+    assert np.allclose(lo.matrix_sqrt(np.diag([4.0, 9.0])), np.diag([2.0, 3.0]), atol=1e-10)
+    assert abs(lo.matrix_sqrt([[16.0]])[0, 0] - 4) < 1e-10
+    assert np.allclose(lo.matrix_sqrt(np.eye(2)), np.eye(2), atol=1e-10)
+    assert np.allclose(lo.matrix_sqrt(np.diag([1.0, -1.0])), np.diag([1.0, 1j]), atol=1e-10)
+    assert abs(lo.spectral_norm(np.eye(2)) - 1) < 1e-12
+    assert abs(lo.spectral_norm([[3.0, 4.0], [0.0, 5.0]]) - 6.708203932499369) < 1e-10
+    assert abs(lo.spectral_norm(np.zeros((2, 2)))) < 1e-12
+    assert abs(lo.condition_number(np.eye(2)) - 1) < 1e-12
+    assert 1 < lo.condition_number([[4.0, 1.0], [1.0, 3.0]]) < 5
+    assert lo.condition_number([[1.0, 1.0], [1.0, 1.001]]) > 100
+    assert all(abs(s - 1) < 1e-10 for s in lo.svd(np.eye(2))["S"])
+    s = lo.svd(np.diag([3.0, 4.0]))["S"]
+    assert len(s) == 2 and s[0].real >= s[1].real and abs(s[0] - 4) < 1e-10 and abs(s[1] - 3) < 1e-10
+    s = lo.svd(np.diag([2.0, 3.0, 1.0]))["S"]
+    assert len(s) == 3 and s[0].real >= s[1].real >= s[2].real
+    A = np.diag([4.0, 2.0])
+    sv = [x.real for x in lo.svd(A)["S"]]
+    assert abs(max(sv) - lo.spectral_norm(A)) < 1e-10
+    assert abs(max(sv) / min(sv) - lo.condition_number(A)) < 1e-8
+
+
+@pytest.mark.parametrize("m,n", [(1, 1), (3, 3), (5, 3), (3, 5), (8, 8)])
+def test_svd_against_lapack(m, n):
+    from scipy.linalg import svdvals
+    rng = np.random.default_rng(100 + 10 * m + n)
+    A = rand_c(rng, m, n)
+    r = lo.svd(A)
+    s = np.array([x.real for x in r["S"]])
+    ref = svdvals(A)
+    assert np.allclose(s[:min(m, n)], ref, atol=1e-10)
+    k = min(m, n)
+    # A = U[:, :k] diag(s[:k]) V†[:k, :]
+    assert np.allclose(r["U"][:, :k] @ np.diag(s[:k]) @ r["V†"][:k, :], A, atol=1e-10)
+    assert np.allclose(r["V†"] @ r["V†"].conj().T, np.eye(n), atol=1e-10)
+    assert abs(lo.spectral_norm(A) - ref[0]) < 1e-10
+
+
+@pytest.mark.parametrize("n", [2, 4, 9])
+def test_matrix_functions_against_scipy(n):
+    from scipy.linalg import expm, logm, sqrtm
+    rng = np.random.default_rng(200 + n)
+    H = rand_c(rng, n, n)
+    H = (H + H.conj().T) / 2
+    assert np.allclose(lo.matrix_exp(H), expm(H), atol=1e-10)
+    assert np.allclose(lo.matrix_exp(1j * H), expm(1j * H), atol=1e-10)
+    G = rand_c(rng, n, n) / n
+    assert np.allclose(lo.matrix_exp(G), expm(G), atol=1e-10)
+    P = H @ H.conj().T + np.eye(n)                     # Hermitian positive definite
+    assert np.allclose(lo.matrix_log(P), logm(P), atol=1e-9)
+    assert np.allclose(lo.matrix_sqrt(P), sqrtm(P), atol=1e-9)
+    assert np.allclose(lo.matrix_exp(lo.matrix_log(P)), P, atol=1e-9)
+    S = lo.matrix_sqrt(P)
+    assert np.allclose(S @ S, P, atol=1e-9)
+    with pytest.raises(lo.OracleLAError):
+        lo.matrix_log(G)
