@@ -1,8 +1,10 @@
 // qsv_fuse.hip — fused tiled passes: many gates per read/write of the state.
 //
-// A pass owns a set of TB = 12 "tile" bits (64 KiB of LDS for complex128, two
-// workgroups per CU; 32 KiB for complex64, three per CU, which their register
-// count allows). The 2^(n-TB) tiles are independent: a workgroup
+// A pass owns a set of TB = 12 "tile" bits. A tile crosses LDS in 32 KiB: a
+// complex64 tile whole, a complex128 tile one plane (real, then imaginary) at
+// a time, so three complex128 workgroups fit a CU (their register count
+// allows three) and four complex64 ones with FP32 arithmetic. The 2^(n-TB)
+// tiles are independent: a workgroup
 // of 2^(TB-4) threads loads one tile, each thread holding 16 amplitudes in
 // registers, applies the pass's gates and stores it back, so the pass moves
 // the state through HBM once however many gates it holds.
