@@ -1415,6 +1415,11 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
   if (!d->num_qubits || !d->op_offsets || !d->term_offsets || (d->op_offsets[N] && !d->ops) ||
       (d->term_offsets[N] && (!d->term_zmask || !d->term_coeff)))
     return fail(QSV_E_INVALID, "batch descriptor has null arrays");
+  // instance i reads ops [op_offsets[i], op_offsets[i+1]) and terms
+  // [term_offsets[i], term_offsets[i+1]): both must be non-decreasing
+  for (uint64_t i = 0; i < N; ++i)
+    if (d->op_offsets[i] > d->op_offsets[i + 1] || d->term_offsets[i] > d->term_offsets[i + 1])
+      return fail(QSV_E_INVALID, "instance " + std::to_string(i) + ": offsets must be non-decreasing");
   int rc = ensure_init();
   if (rc) return rc;
   HIPCHK(hipSetDevice(g_device));

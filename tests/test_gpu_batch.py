@@ -97,3 +97,24 @@ def test_grouped_passes_uneven_programs(eng, oracle, prec, tol):
         psi = oracle.run(n, ops)
         exp = sum(c * oracle.pauli_expectation(psi, n, s) for c, s in terms)
         assert abs(g - exp) < tol * max(1.0, sum(abs(c) for c, _ in terms)), (n, len(ops), g, exp)
+
+
+def test_batch_rejects_decreasing_offsets(eng):
+    """An instance's op or term range [off[i], off[i+1]) must not run backwards:
+    the call fails before reading past the arrays, and the next call works."""
+    import ctypes
+    from qclojure_amd import _abi
+    nq = np.array([2, 2], dtype=np.int32)
+    ops = qc.ops_from_tuples([(_abi.OP["H"], 0, 0, 0, 0, 0.0), (_abi.OP["CNOT"], 0, 1, 0, 0, 0.0)])
+    zm = np.array([1, 2], dtype=np.uint64)
+    co = np.array([1.0, 1.0])
+    out = np.zeros(2)
+    for opoff, toff in (([0, 2, 1], [0, 1, 2]), ([0, 1, 2], [0, 2, 1])):
+        opo = np.array(opoff, dtype=np.uint64)
+        to = np.array(toff, dtype=np.uint64)
+        desc = _abi.QsvBatchDesc(2, nq.ctypes.data, opo.ctypes.data, ops.ctypes.data, to.ctypes.data,
+                                 zm.ctypes.data, co.ctypes.data, _abi.QSV_C128)
+        with pytest.raises(_abi.QsvError, match="non-decreasing"):
+            _abi.check(eng.lib().qsv_batch_expect_z(ctypes.byref(desc), out.ctypes.data))
+    e = eng.batch_expect_z([(2, ops, [(1.0, "ZI"), (1.0, "IZ")])], "c128")
+    assert abs(e[0]) < 1e-12
