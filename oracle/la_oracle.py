@@ -318,7 +318,11 @@ def _diag_or_hermitian(A, f, what):
             raise OracleLAError(f"Cannot compute {what} of zero diagonal element")
         return np.diag([f(z) for z in d]).astype(complex)
     if not hermitian(A, 1e-10):
-        raise OracleLAError(f"{what} of a non-Hermitian matrix is not restated")
+        # the principal function of a general matrix, as scipy.linalg computes it
+        from scipy.linalg import logm, sqrtm
+        if abs(np.linalg.det(A)) < 1e-300:
+            raise OracleLAError(f"Cannot compute {what} of a singular matrix")
+        return logm(A) if what == "log" else sqrtm(A)
     w = np.linalg.eigvalsh(A)
     if np.any(np.abs(w) < 1e-14):
         raise OracleLAError(f"Cannot compute {what} of zero eigenvalue")

@@ -5,7 +5,7 @@ complex_linear_algebra.clj).
 ``HipBackend`` implements BackendAdapter (protocols.clj:12-76), MatrixAlgebra
 (protocols.clj:81-353) and eigen-hermitian (protocols.clj:363-373) with one
 libqsv call per operation, and on top of those device calls svd, the matrix
-functions (exp; log and sqrt of diagonal and Hermitian matrices) and
+functions (exp, log, sqrt) and
 MatrixAnalysis (protocols.clj:357-520); eigen-general, LU, QR and Cholesky
 raise LinearAlgebraError; its matrices are opaque device handles
 (``DeviceMatrix``), which the protocol allows ("implementations may return
@@ -589,10 +589,6 @@ class HipBackend:
         return DeviceMatrix.from_numpy(np.diag([f(z) for z in d]).astype(np.complex128))
 
     def _hermitian_fn(self, A: DeviceMatrix, f, what: str, failed: str):
-        if not self.hermitian(A, 1e-10):
-            raise LinearAlgebraError(failed, {"original-error": "a non-Hermitian, non-diagonal matrix needs "
-                                              "eigen-general, which the :hip backend does not provide",
-                                              "matrix-size": A.rows})
         w, V = self._eigh_dev(A)
         if np.any(np.abs(w) < 1e-14):
             raise LinearAlgebraError(failed, {"original-error": f"Cannot compute {what} of zero eigenvalue",
@@ -610,23 +606,71 @@ class HipBackend:
         r, th = np.sqrt(abs(z)), np.angle(z) / 2.0
         return complex(r * np.cos(th), r * np.sin(th))
 
+    def _db_sqrt(self, A: DeviceMatrix, failed: str) -> DeviceMatrix:
+        """Principal square root of a general matrix by the Denman-Beavers
+        iteration Y <- (Y + Z^-1)/2, Z <- (Z + Y^-1)/2 from Y = A, Z = I
+        (device inverses, sums and norms). Fails for a singular A or one with
+        eigenvalues on the closed negative real axis, where it does not settle."""
+        n = A.rows
+        Y, Z = A, DeviceMatrix.from_numpy(np.eye(n, dtype=np.complex128))
+        for _ in range(60):
+            try:
+                Yi, Zi = self.inverse(Y), self.inverse(Z)
+            except LinearAlgebraError as e:
+                raise LinearAlgebraError(failed, {"original-error": str(e), "matrix-size": n}) from None
+            Yn = self._axpby(0.5 + 0j, Y, 0.5 + 0j, Zi)
+            Z = self._axpby(0.5 + 0j, Z, 0.5 + 0j, Yi)
+            step = self.norm2(self.subtract(Yn, Y))
+            Y = Yn
+            if step <= 1e-14 * self.norm2(Y):
+                return Y
+        raise LinearAlgebraError(failed, {"original-error": "the square-root iteration did not converge "
+                                          "(an eigenvalue on the closed negative real axis)", "matrix-size": n})
+
+    def _iss_log(self, A: DeviceMatrix, failed: str) -> DeviceMatrix:
+        """Principal logarithm of a general matrix by inverse scaling and
+        squaring: k square roots until R = A^(1/2^k) is within 1/4 of I, then
+        2^k log(I + X), X = R - I, from its Taylor series (all on the device)."""
+        n = A.rows
+        I = DeviceMatrix.from_numpy(np.eye(n, dtype=np.complex128))
+        R, k = A, 0
+        while self.norm2(self.subtract(R, I)) > 0.25:
+            if k == 40:
+                raise LinearAlgebraError(failed, {"original-error": "no convergence", "matrix-size": n})
+            R = self._db_sqrt(R, failed)
+            k += 1
+        X = self.subtract(R, I)
+        L, term = X, X
+        for j in range(2, 60):
+            term = self._gemm(term, X)
+            c = (-1.0) ** (j + 1) / j
+            L = self._axpby(1 + 0j, L, complex(c), term)
+            if abs(c) * self.norm2(term) <= 1e-17 * self.norm2(L):
+                break
+        return self.scale(L, float(2 ** k))
+
     def matrix_log(self, A):
         """matrix-log (:1260-1329): principal branch, log|λ| + i arg λ; a
         diagonal A (1e-10) entry by entry, a Hermitian A through its
-        eigensystem."""
+        eigensystem, any other A by inverse scaling and squaring (where the
+        reference goes through eigen-general, mostly failing)."""
         A = self._square(A, "Matrix logarithm")
         if self.diagonal(A, 1e-10):
             return self._diag_fn(A, self._log, "log")
-        return self._hermitian_fn(A, self._log, "log", "Complex matrix logarithm failed")
+        if self.hermitian(A, 1e-10):
+            return self._hermitian_fn(A, self._log, "log", "Complex matrix logarithm failed")
+        return self._iss_log(A, "Complex matrix logarithm failed")
 
     def matrix_sqrt(self, A):
         """matrix-sqrt (:1331-1407): principal branch, sqrt|λ| e^{i arg(λ)/2};
         a diagonal A (1e-10) entry by entry, a Hermitian A through its
-        eigensystem."""
+        eigensystem, any other A by the Denman-Beavers iteration."""
         A = self._square(A, "Matrix square root")
         if self.diagonal(A, 1e-10):
             return self._diag_fn(A, self._sqrt, "sqrt")
-        return self._hermitian_fn(A, self._sqrt, "sqrt", "Complex matrix square root failed")
+        if self.hermitian(A, 1e-10):
+            return self._hermitian_fn(A, self._sqrt, "sqrt", "Complex matrix square root failed")
+        return self._db_sqrt(A, "Complex matrix square root failed")
 
 
 # ---- facade (complex_linear_algebra.clj:18-740) ----------------------------------------------

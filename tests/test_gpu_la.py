@@ -268,7 +268,7 @@ def test_reference_function_cases(hip):
         with pytest.raises(la.LinearAlgebraError):
             la.eigen_general(np.eye(2))
         with pytest.raises(la.LinearAlgebraError):
-            la.matrix_log(np.array([[1.0, 2.0], [0.5, 1.0]]))
+            la.matrix_log(np.array([[1.0, 2.0], [0.5, 1.0]]))   # singular
 
 
 @pytest.mark.parametrize("m,n", [(1, 1), (3, 3), (7, 4), (4, 7), (64, 64), (90, 70)])
@@ -313,3 +313,8 @@ def test_matrix_functions(hip, n):
     P = H @ H.conj().T / n + np.eye(n)
     assert np.allclose(A_(hip.matrix_log(P)), lo.matrix_log(P), atol=tol)
     assert np.allclose(A_(hip.matrix_sqrt(P)), lo.matrix_sqrt(P), atol=tol)
+    # general (non-normal) matrices: inverse scaling and squaring / Denman-Beavers
+    Gp = G + 3 * np.eye(n)
+    assert np.allclose(A_(hip.matrix_log(Gp)), lo.matrix_log(Gp), atol=tol)
+    S = A_(hip.matrix_sqrt(Gp))
+    assert np.allclose(S, lo.matrix_sqrt(Gp), atol=tol) and np.allclose(S @ S, Gp, atol=tol)

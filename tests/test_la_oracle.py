@@ -165,5 +165,8 @@ def test_matrix_functions_against_scipy(n):
     assert np.allclose(lo.matrix_exp(lo.matrix_log(P)), P, atol=1e-9)
     S = lo.matrix_sqrt(P)
     assert np.allclose(S @ S, P, atol=1e-9)
+    Gp = G + 3 * np.eye(n)                               # eigenvalues off the negative real axis
+    assert np.allclose(lo.matrix_log(Gp), logm(Gp), atol=1e-10)
+    assert np.allclose(lo.matrix_sqrt(Gp) @ lo.matrix_sqrt(Gp), Gp, atol=1e-10)
     with pytest.raises(lo.OracleLAError):
-        lo.matrix_log(G)
+        lo.matrix_log(np.array([[1.0, 2.0], [0.5, 1.0]]))   # singular
