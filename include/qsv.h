@@ -358,6 +358,23 @@ int qsv_la_eigh(const qsv_mat* A, double* eigenvalues, qsv_mat* V, int* sweeps);
  * reference builds in expand-gate-to-n-qubits (src/org/soulspace/qclojure/domain/gate.clj:323-353),
  * applied in O(2^n) per factor instead of through the 2^n x 2^n matrix. */
 int qsv_la_kron_apply(const double* const* factors, int nfactors, qsv_mat* x);
+/* cholesky-decomposition (src/org/soulspace/qclojure/domain/math/protocols.clj:431;
+ * fastmath/complex_linear_algebra.clj:1104-1166): L lower triangular with
+ * A = L L^H, from A's lower triangle; the caller checks hermitian? first.
+ * QSV_E_INVALID "Hermitian matrix diagonal must be real" when a diagonal
+ * residual has |imag| > eps, "Matrix is not positive definite" when its real
+ * part is below eps. */
+int qsv_la_cholesky(const qsv_mat* A, double eps, qsv_mat* L);
+/* lu-decomposition (protocols.clj:403; fastmath/complex_linear_algebra.clj:965-1046):
+ * P A = L U with partial pivoting (the first row of largest |a_ik|), L unit
+ * lower, U upper; perm[k] = the row of A that is row k of P A (n entries).
+ * QSV_E_INVALID "Matrix is singular" for a pivot below 1e-11. */
+int qsv_la_lu(const qsv_mat* A, qsv_mat* L, qsv_mat* U, int64_t* perm);
+/* qr-decomposition (protocols.clj:416; fastmath/complex_linear_algebra.clj:1048-1102):
+ * modified Gram-Schmidt, A (m x n) = Q (m x n) R (n x n), R upper triangular
+ * with real diagonal ||v_j||; a column whose residual norm is below 1e-12 gets
+ * the unit vector e_j in Q (zero when j >= m). */
+int qsv_la_qr(const qsv_mat* A, qsv_mat* Q, qsv_mat* R);
 
 #ifdef __cplusplus
 }

@@ -337,3 +337,82 @@ def matrix_log(A):
 def matrix_sqrt(A):
     """matrix-sqrt (:1331-1407): principal square root, diagonal entry-wise."""
     return _diag_or_hermitian(A, _principal_sqrt, "sqrt")
+
+
+# ---- LU, QR, Cholesky ----------------------------------------------------------------------
+
+def cholesky(A, eps=1e-12):
+    """cholesky-decomposition (:1104-1166), row by row: L[i][j] = (A[i][j] -
+    sum_{k<j} L[i][k] conj(L[j][k])) / L[j][j]; the diagonal residual must be
+    real (|imag| <= eps) and at least eps."""
+    A = np.asarray(A, complex)
+    n, m = A.shape
+    if n != m:
+        raise OracleLAError("Cholesky decomposition requires square matrix")
+    if not hermitian(A, eps):
+        raise OracleLAError("Cholesky decomposition requires Hermitian matrix")
+    L = np.zeros((n, n), complex)
+    for i in range(n):
+        for j in range(i + 1):
+            s = 0j
+            for k in range(j):
+                s += L[i, k] * np.conj(L[j, k])
+            diff = A[i, j] - s
+            if i == j:
+                if abs(diff.imag) > eps:
+                    raise OracleLAError("Hermitian matrix diagonal must be real")
+                if diff.real < eps:
+                    raise OracleLAError("Matrix is not positive definite")
+                L[i, i] = np.sqrt(diff.real)
+            else:
+                L[i, j] = diff / L[j, j]
+    return {"L": L}
+
+
+def lu(A):
+    """lu-decomposition (:965-1046) for the factors the reference gets on real
+    input: P A = L U by partial pivoting (first row of largest |a_ik|), pivots
+    below 1e-11 singular (Apache Commons Math's default threshold)."""
+    M = np.array(A, dtype=complex)
+    n, m = M.shape
+    if n != m:
+        raise OracleLAError("LU decomposition requires square matrix")
+    order = list(range(n))
+    for k in range(n):
+        col = np.abs(M[k:, k])
+        p = k + int(np.argmax(col))          # argmax returns the first of equal values
+        if not col[p - k] >= 1e-11:
+            raise OracleLAError("Matrix is singular")
+        if p != k:
+            M[[k, p]] = M[[p, k]]
+            order[k], order[p] = order[p], order[k]
+        M[k + 1:, k] /= M[k, k]
+        M[k + 1:, k + 1:] -= np.outer(M[k + 1:, k], M[k, k + 1:])
+    L = np.tril(M, -1) + np.eye(n)
+    U = np.triu(M)
+    P = np.zeros((n, n), complex)
+    P[np.arange(n), order] = 1.0
+    return {"L": L, "U": U, "P": P}
+
+
+def qr(A):
+    """qr-decomposition (:1048-1102): modified Gram-Schmidt; Q is m x n, R n x n;
+    a residual below 1e-12 gives the unit vector e_j."""
+    A = np.asarray(A, complex)
+    m, n = A.shape
+    Q = np.zeros((m, n), complex)
+    R = np.zeros((n, n), complex)
+    for j in range(n):
+        v = A[:, j].copy()
+        for i in range(j):
+            R[i, j] = np.sum(np.conj(Q[:, i]) * v)
+            v = v - R[i, j] * Q[:, i]
+        nrm = float(np.sqrt(np.sum((np.conj(v) * v).real)))
+        R[j, j] = nrm
+        if nrm < 1e-12:
+            Q[:, j] = 0
+            if j < m:
+                Q[j, j] = 1.0
+        else:
+            Q[:, j] = v / nrm
+    return {"Q": Q, "R": R}

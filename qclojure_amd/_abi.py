@@ -177,6 +177,9 @@ SIGNATURES = {
     "qsv_la_solve": (_i32, [_vp, _vp, _vp]),
     "qsv_la_eigh": (_i32, [_vp, _pd, _vp, ctypes.POINTER(_i32)]),
     "qsv_la_kron_apply": (_i32, [ctypes.POINTER(_pd), _i32, _vp]),
+    "qsv_la_cholesky": (_i32, [_vp, _dbl, _vp]),
+    "qsv_la_lu": (_i32, [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_int64)]),
+    "qsv_la_qr": (_i32, [_vp, _vp, _vp]),
 }
 
 QSV_LA_DEV_HERMITIAN, QSV_LA_DEV_OFFDIAG, QSV_LA_DEV_IDENTITY = 0, 1, 2

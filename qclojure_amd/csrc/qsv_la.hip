@@ -740,6 +740,177 @@ int alloc(double2** p, uint64_t count)
 
 }  // namespace
 
+// ---- LU, QR and Cholesky: one workgroup each ----------------------------------------------
+// The factorizations are column-sequential, so each runs as a single
+// workgroup that loops over the columns with barriers between the steps
+// (every intermediate in HBM/L2); meant for the facade's matrix sizes.
+constexpr int kDecBlock = 1024;
+
+// Sum of v over the workgroup, in a fixed order; every thread gets it.
+__device__ double2 wg_sum(double2 v, double2* red)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    v.x += __shfl_xor(v.x, o);
+    v.y += __shfl_xor(v.y, o);
+  }
+  __syncthreads();   // red may still be read from the previous call
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double2 s = make_double2(0.0, 0.0);
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s = cadd(s, red[w]);
+  return s;
+}
+
+// Cholesky A = L Lᴴ as fastmath/complex_linear_algebra.clj:1119-1166 defines
+// it: L[i][j] = (A[i][j] - Σ_{k<j} L[i][k] conj(L[j][k])) / L[j][j] below the
+// diagonal, L[j][j] = sqrt(A[j][j] - Σ_k |L[j][k]|^2), which must be real
+// (|imag| <= eps, else status 1) and at least eps (else status 2); status[1]
+// = the failing index. Column by column: the first failing diagonal is the
+// one the reference's row-by-row order reaches first.
+__global__ __launch_bounds__(kDecBlock) void k_cholesky(const double2* __restrict__ A, double2* __restrict__ L,
+                                                        int n, double eps, int* __restrict__ status)
+{
+  __shared__ double2 red[kDecBlock / 64];
+  __shared__ double2 djj;
+  __shared__ int bad;
+  const int t = threadIdx.x, nt = blockDim.x;
+  for (int64_t e = t; e < (int64_t)n * n; e += nt) L[e] = make_double2(0.0, 0.0);
+  __syncthreads();
+  for (int j = 0; j < n; ++j) {
+    double2 s = make_double2(0.0, 0.0);
+    for (int k = t; k < j; k += nt) s.x += cabs2(L[(int64_t)j * n + k]);
+    s = wg_sum(s, red);
+    if (t == 0) {
+      const double2 diff = csub(A[(int64_t)j * n + j], s);
+      int b = 0;
+      if (fabs(diff.y) > eps) b = 1;
+      else if (diff.x < eps) b = 2;
+      if (b) {
+        status[0] = b;
+        status[1] = j;
+      } else {
+        djj = make_double2(sqrt(diff.x), 0.0);
+        L[(int64_t)j * n + j] = djj;
+      }
+      bad = b;
+    }
+    __syncthreads();
+    if (bad) return;
+    for (int i = j + 1 + t; i < n; i += nt) {
+      double2 acc = make_double2(0.0, 0.0);
+      for (int k = 0; k < j; ++k) acc = cadd(acc, cmul(L[(int64_t)i * n + k], cconj(L[(int64_t)j * n + k])));
+      L[(int64_t)i * n + j] = cdiv(csub(A[(int64_t)i * n + j], acc), djj);
+    }
+    __syncthreads();
+  }
+}
+
+// LU with partial pivoting on M (n x n, overwritten: multipliers below the
+// diagonal, U on and above it): the pivot of column k is the first row of
+// largest |M[i][k]|, i >= k; perm[k] = the row swapped into row k. A pivot
+// below 1e-11 (the singularity threshold of Apache Commons Math's
+// LUDecomposition, which the reference calls) sets status[0] = 1, status[1] = k.
+__global__ __launch_bounds__(kDecBlock) void k_lu(double2* __restrict__ M, int n, int* __restrict__ perm,
+                                                  int* __restrict__ status)
+{
+  __shared__ double bv[kDecBlock / 64];
+  __shared__ int bi[kDecBlock / 64];
+  __shared__ int prow;
+  __shared__ int bad;
+  const int t = threadIdx.x, nt = blockDim.x;
+  for (int k = 0; k < n; ++k) {
+    double v = -1.0;
+    int idx = n;
+    for (int i = k + t; i < n; i += nt) {
+      const double a = cabs_(M[(int64_t)i * n + k]);
+      if (a > v) { v = a; idx = i; }   // rows ascending per thread: the first of equal values stays
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(v, o);
+      const int oi = __shfl_xor(idx, o);
+      if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+    }
+    if ((t & 63) == 0) { bv[t >> 6] = v; bi[t >> 6] = idx; }
+    __syncthreads();
+    if (t == 0) {
+      double bvv = -1.0;
+      int bii = n;
+      for (int w = 0; w < nt / 64; ++w)
+        if (bv[w] > bvv || (bv[w] == bvv && bi[w] < bii)) { bvv = bv[w]; bii = bi[w]; }
+      bad = !(bvv >= 1e-11);
+      if (bad) { status[0] = 1; status[1] = k; }
+      prow = bii;
+      perm[k] = bii;
+    }
+    __syncthreads();
+    if (bad) return;
+    const int p = prow;
+    if (p != k)
+      for (int j = t; j < n; j += nt) {
+        const double2 a = M[(int64_t)k * n + j];
+        M[(int64_t)k * n + j] = M[(int64_t)p * n + j];
+        M[(int64_t)p * n + j] = a;
+      }
+    __syncthreads();
+    const double2 piv = M[(int64_t)k * n + k];
+    for (int i = k + 1 + t; i < n; i += nt) M[(int64_t)i * n + k] = cdiv(M[(int64_t)i * n + k], piv);
+    __syncthreads();
+    const int64_t w = n - k - 1;
+    for (int64_t e = t; e < w * w; e += nt) {
+      const int64_t i = k + 1 + e / w, j = k + 1 + e % w;
+      M[i * n + j] = csub(M[i * n + j], cmul(M[i * n + k], M[(int64_t)k * n + j]));
+    }
+    __syncthreads();
+  }
+}
+
+// L (unit lower) and U (upper) out of the overwritten LU matrix.
+__global__ void k_lu_split(const double2* __restrict__ M, double2* __restrict__ L, double2* __restrict__ U, int64_t n)
+{
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / n, j = e % n;
+    const double2 z = make_double2(0.0, 0.0);
+    L[e] = i > j ? M[e] : (i == j ? make_double2(1.0, 0.0) : z);
+    U[e] = i <= j ? M[e] : z;
+  }
+}
+
+// QR by modified Gram-Schmidt as fastmath/complex_linear_algebra.clj:1048-1102:
+// for each column j, v = A[:, j]; for i < j: R[i][j] = <q_i, v> (conjugating
+// q_i), v -= R[i][j] q_i; R[j][j] = ||v||, q_j = v / ||v||, or the unit vector
+// e_j (zero when j >= m) when ||v|| < 1e-12. Q is m x n, R is n x n.
+__global__ __launch_bounds__(kDecBlock) void k_qr_mgs(const double2* __restrict__ A, double2* __restrict__ Q,
+                                                      double2* __restrict__ R, int m, int n)
+{
+  __shared__ double2 red[kDecBlock / 64];
+  const int t = threadIdx.x, nt = blockDim.x;
+  for (int64_t e = t; e < (int64_t)n * n; e += nt) R[e] = make_double2(0.0, 0.0);
+  for (int j = 0; j < n; ++j) {
+    for (int r = t; r < m; r += nt) Q[(int64_t)r * n + j] = A[(int64_t)r * n + j];
+    __syncthreads();
+    for (int i = 0; i < j; ++i) {
+      double2 s = make_double2(0.0, 0.0);
+      for (int r = t; r < m; r += nt) s = cadd(s, cmul(cconj(Q[(int64_t)r * n + i]), Q[(int64_t)r * n + j]));
+      s = wg_sum(s, red);
+      if (t == 0) R[(int64_t)i * n + j] = s;
+      for (int r = t; r < m; r += nt) Q[(int64_t)r * n + j] = csub(Q[(int64_t)r * n + j], cmul(s, Q[(int64_t)r * n + i]));
+      __syncthreads();
+    }
+    double2 s = make_double2(0.0, 0.0);
+    for (int r = t; r < m; r += nt) s.x += cabs2(Q[(int64_t)r * n + j]);
+    s = wg_sum(s, red);
+    const double nrm = sqrt(s.x);
+    if (t == 0) R[(int64_t)j * n + j] = make_double2(nrm, 0.0);
+    for (int r = t; r < m; r += nt) {
+      const int64_t e = (int64_t)r * n + j;
+      Q[e] = nrm < 1e-12 ? make_double2(r == j ? 1.0 : 0.0, 0.0) : cdiv(Q[e], make_double2(nrm, 0.0));
+    }
+    __syncthreads();
+  }
+}
+
 extern "C" {
 
 int qsv_mat_create(int64_t rows, int64_t cols, qsv_mat** out)
@@ -1181,6 +1352,95 @@ int qsv_la_kron_apply(const double* const* factors, int nfactors, qsv_mat* x)
                        make_double2(g[0], g[1]), make_double2(g[2], g[3]), make_double2(g[4], g[5]),
                        make_double2(g[6], g[7]));
   }
+  LACHK(hipGetLastError());
+  LACHK(hipStreamSynchronize(g_la_stream));
+  return QSV_OK;
+}
+
+int qsv_la_cholesky(const qsv_mat* A, double eps, qsv_mat* L)
+{
+  int rc = check_mat(A, "A");
+  if (rc || (rc = check_mat(L, "L"))) return rc;
+  const int64_t n = A->rows;
+  if (A->cols != n) return api_fail(QSV_E_INVALID, "Cholesky decomposition requires square matrix");
+  if (L->rows != n || L->cols != n) return api_fail(QSV_E_INVALID, "cholesky-decomposition: L shape differs");
+  if (L->data == A->data) return api_fail(QSV_E_INVALID, "cholesky-decomposition: L aliases A");
+  if (n > 8192) return api_fail(QSV_E_INVALID, "cholesky-decomposition: matrix too large");
+  std::lock_guard<std::mutex> lk(g_la_mu);
+  int st[2] = {0, 0};
+  LACHK(hipMemsetAsync(g_la_flag, 0, 2 * sizeof(int), g_la_stream));
+  hipLaunchKernelGGL(k_cholesky, dim3(1), dim3(kDecBlock), 0, g_la_stream, (const double2*)A->data, L->data, (int)n,
+                     eps, g_la_flag);
+  LACHK(hipGetLastError());
+  LACHK(hipMemcpyAsync(st, g_la_flag, 2 * sizeof(int), hipMemcpyDeviceToHost, g_la_stream));
+  LACHK(hipStreamSynchronize(g_la_stream));
+  if (st[0] == 1)
+    return api_fail(QSV_E_INVALID, ("Hermitian matrix diagonal must be real (i = " + std::to_string(st[1]) + ")").c_str());
+  if (st[0] == 2)
+    return api_fail(QSV_E_INVALID, ("Matrix is not positive definite (i = " + std::to_string(st[1]) + ")").c_str());
+  return QSV_OK;
+}
+
+int qsv_la_lu(const qsv_mat* A, qsv_mat* L, qsv_mat* U, int64_t* perm)
+{
+  int rc = check_mat(A, "A");
+  if (rc || (rc = check_mat(L, "L")) || (rc = check_mat(U, "U"))) return rc;
+  const int64_t n = A->rows;
+  if (A->cols != n) return api_fail(QSV_E_INVALID, "LU decomposition requires square matrix");
+  if (L->rows != n || L->cols != n || U->rows != n || U->cols != n)
+    return api_fail(QSV_E_INVALID, "lu-decomposition: L or U shape differs");
+  if (!perm) return api_fail(QSV_E_INVALID, "null permutation buffer");
+  if (n > 8192) return api_fail(QSV_E_INVALID, "lu-decomposition: matrix too large");
+  std::lock_guard<std::mutex> lk(g_la_mu);
+  double2* M = nullptr;
+  int* dperm = nullptr;
+  if (hipMalloc(&M, (size_t)n * n * sizeof(double2)) != hipSuccess ||
+      hipMalloc(&dperm, (size_t)std::max<int64_t>(n, 1) * sizeof(int)) != hipSuccess) {
+    hipFree(M);
+    return api_fail(QSV_E_NOMEM, "lu-decomposition scratch allocation failed");
+  }
+  std::vector<int> pv((size_t)n);
+  int st[2] = {0, 0};
+  bool ok = hipMemcpyAsync(M, A->data, (size_t)n * n * sizeof(double2), hipMemcpyDeviceToDevice, g_la_stream) ==
+                hipSuccess &&
+            hipMemsetAsync(g_la_flag, 0, 2 * sizeof(int), g_la_stream) == hipSuccess;
+  if (ok) {
+    hipLaunchKernelGGL(k_lu, dim3(1), dim3(kDecBlock), 0, g_la_stream, M, (int)n, dperm, g_la_flag);
+    ok = hipGetLastError() == hipSuccess;
+  }
+  if (ok) {
+    hipLaunchKernelGGL(k_lu_split, dim3(grid_for((uint64_t)n * n)), dim3(kLaBlock), 0, g_la_stream,
+                       (const double2*)M, L->data, U->data, n);
+    ok = hipGetLastError() == hipSuccess;
+  }
+  ok = ok && hipMemcpyAsync(st, g_la_flag, 2 * sizeof(int), hipMemcpyDeviceToHost, g_la_stream) == hipSuccess &&
+       (n == 0 ||
+        hipMemcpyAsync(pv.data(), dperm, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, g_la_stream) == hipSuccess);
+  ok = hipStreamSynchronize(g_la_stream) == hipSuccess && ok;
+  hipFree(M);
+  hipFree(dperm);
+  if (!ok) return api_fail(QSV_E_DEVICE, "lu-decomposition failed");
+  if (st[0]) return api_fail(QSV_E_INVALID, ("Matrix is singular (column " + std::to_string(st[1]) + ")").c_str());
+  // row k of P·A is row perm[k] of A: apply the swaps in order to the identity order
+  std::vector<int64_t> order((size_t)n);
+  for (int64_t i = 0; i < n; ++i) order[i] = i;
+  for (int64_t k = 0; k < n; ++k) std::swap(order[k], order[pv[k]]);
+  for (int64_t k = 0; k < n; ++k) perm[k] = order[k];
+  return QSV_OK;
+}
+
+int qsv_la_qr(const qsv_mat* A, qsv_mat* Q, qsv_mat* R)
+{
+  int rc = check_mat(A, "A");
+  if (rc || (rc = check_mat(Q, "Q")) || (rc = check_mat(R, "R"))) return rc;
+  const int64_t m = A->rows, n = A->cols;
+  if (Q->rows != m || Q->cols != n || R->rows != n || R->cols != n)
+    return api_fail(QSV_E_INVALID, "qr-decomposition: Q or R shape differs");
+  if (Q->data == A->data || R->data == A->data) return api_fail(QSV_E_INVALID, "qr-decomposition: output aliases A");
+  if (m > (1 << 20) || n > 8192) return api_fail(QSV_E_INVALID, "qr-decomposition: matrix too large");
+  std::lock_guard<std::mutex> lk(g_la_mu);
+  hipLaunchKernelGGL(k_qr_mgs, dim3(1), dim3(kDecBlock), 0, g_la_stream, (const double2*)A->data, Q->data, R->data,
+                     (int)m, (int)n);
   LACHK(hipGetLastError());
   LACHK(hipStreamSynchronize(g_la_stream));
   return QSV_OK;

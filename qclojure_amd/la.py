@@ -6,8 +6,8 @@ complex_linear_algebra.clj).
 (protocols.clj:81-353) and eigen-hermitian (protocols.clj:363-373) with one
 libqsv call per operation, and on top of those device calls svd, the matrix
 functions (exp, log, sqrt) and
-MatrixAnalysis (protocols.clj:357-520); eigen-general, LU, QR and Cholesky
-raise LinearAlgebraError; its matrices are opaque device handles
+MatrixAnalysis, and LU, QR and Cholesky as single-workgroup kernels
+(protocols.clj:357-520); eigen-general raises LinearAlgebraError; its matrices are opaque device handles
 (``DeviceMatrix``), which the protocol allows ("implementations may return
 opaque handles"), so chained operations stay in HBM and only ``to_numpy()``
 reads them back. ``kronecker-product`` results are kept as their factors
@@ -537,21 +537,44 @@ class HipBackend:
             U = self.add(U, DeviceMatrix.from_numpy(E))
         return {"U": U, "S": [complex(sv[j], 0.0) for j in order], "V†": self.conjugate_transpose(Vs)}
 
-    def _not_provided(self, name: str):
-        raise LinearAlgebraError(f"{name} is not provided by the :hip backend",
-                                 {"provided": ["eigen-hermitian", "svd"]})
-
     def eigen_general(self, A):
-        self._not_provided("eigen-general")
+        raise LinearAlgebraError("eigen-general is not provided by the :hip backend",
+                                 {"provided": ["eigen-hermitian", "svd", "lu-decomposition", "qr-decomposition",
+                                               "cholesky-decomposition"]})
 
-    def lu_decomposition(self, A):
-        self._not_provided("lu-decomposition")
+    def lu_decomposition(self, A) -> dict:
+        """lu-decomposition (:965-1046): {"L", "U", "P"} with P·A = L·U,
+        partial pivoting on the first row of largest |a_ik| (the pivot rule of
+        Apache Commons Math's LUDecomposition, which the reference applies to
+        the real embedding of A: the same factors for real A), L unit lower."""
+        A = self._square(A, "LU decomposition")
+        n = A.rows
+        L, U = DeviceMatrix(n, n), DeviceMatrix(n, n)
+        perm = np.zeros(max(n, 1), dtype=np.int64)
+        _call(_lib().qsv_la_lu, A.handle, L.handle, U.handle, perm.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+        P = np.zeros((n, n), dtype=np.complex128)
+        P[np.arange(n), perm[:n]] = 1.0
+        return {"L": L, "U": U, "P": DeviceMatrix.from_numpy(P)}
 
-    def qr_decomposition(self, A):
-        self._not_provided("qr-decomposition")
+    def qr_decomposition(self, A) -> dict:
+        """qr-decomposition (:1048-1102): modified Gram-Schmidt, {"Q": m x n,
+        "R": n x n}."""
+        A = self._dense(A)
+        Q, R = DeviceMatrix(A.rows, A.cols), DeviceMatrix(A.cols, A.cols)
+        _call(_lib().qsv_la_qr, A.handle, Q.handle, R.handle)
+        return {"Q": Q, "R": R}
 
-    def cholesky_decomposition(self, A, eps: float | None = None):
-        self._not_provided("cholesky-decomposition")
+    def cholesky_decomposition(self, A, eps: float | None = None) -> dict:
+        """cholesky-decomposition (:1104-1166): {"L"} with A = L Lᴴ; the
+        reference's checks and messages (square, Hermitian within eps, real
+        positive diagonal residuals)."""
+        eps = self.tolerance if eps is None else float(eps)
+        A = self._square(A, "Cholesky decomposition")
+        if not self.hermitian(A, eps):
+            raise LinearAlgebraError("Cholesky decomposition requires Hermitian matrix")
+        L = DeviceMatrix(A.rows, A.rows)
+        _call(_lib().qsv_la_cholesky, A.handle, eps, L.handle)
+        return {"L": L}
 
     # -- MatrixFunctions (protocols.clj:446-492)
     def matrix_exp(self, A):

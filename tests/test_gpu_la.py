@@ -318,3 +318,49 @@ def test_matrix_functions(hip, n):
     assert np.allclose(A_(hip.matrix_log(Gp)), lo.matrix_log(Gp), atol=tol)
     S = A_(hip.matrix_sqrt(Gp))
     assert np.allclose(S, lo.matrix_sqrt(Gp), atol=tol) and np.allclose(S @ S, Gp, atol=tol)
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 64, 130])
+def test_cholesky_lu_qr(hip, n):
+    note(f"la: cholesky / lu / qr n={n}")
+    rng = np.random.default_rng(600 + n)
+    B = rand_c(rng, n, n)
+    P = B @ B.conj().T + n * np.eye(n)
+    P = (P + P.conj().T) / 2
+    L = A_(hip.cholesky_decomposition(P)["L"])
+    assert np.allclose(L, lo.cholesky(P)["L"], atol=1e-12 * n)
+    assert np.allclose(L @ L.conj().T, P, atol=1e-10 * n)
+    R = rng.standard_normal((n, n))
+    for M in (R, B):
+        f = hip.lu_decomposition(M)
+        o = lo.lu(M)
+        Pm, Lm, Um = A_(f["P"]), A_(f["L"]), A_(f["U"])
+        assert np.array_equal(Pm, o["P"])
+        assert np.allclose(Lm, o["L"], atol=1e-11 * n) and np.allclose(Um, o["U"], atol=1e-11 * n)
+        assert np.allclose(Pm @ M, Lm @ Um, atol=1e-11 * n)
+    A = rand_c(rng, n + 3, n)
+    q = hip.qr_decomposition(A)
+    Q, Rq = A_(q["Q"]), A_(q["R"])
+    o = lo.qr(A)
+    assert Q.shape == (n + 3, n) and Rq.shape == (n, n)
+    assert np.allclose(Q, o["Q"], atol=1e-11 * n) and np.allclose(Rq, o["R"], atol=1e-11 * n)
+    assert np.allclose(Q @ Rq, A, atol=1e-11 * n)
+
+
+def test_decomposition_errors(hip):
+    note("la: decomposition errors")
+    with pytest.raises(la.LinearAlgebraError, match="Hermitian"):
+        hip.cholesky_decomposition(np.array([[1, 2], [0, 1]], complex))
+    with pytest.raises(la.LinearAlgebraError, match="positive definite"):
+        hip.cholesky_decomposition(np.array([[1, 2], [2, 1]], complex))
+    with pytest.raises(la.LinearAlgebraError, match="singular"):
+        hip.lu_decomposition(np.array([[1, 2], [2, 4]], complex))
+    with pytest.raises(la.LinearAlgebraError, match="square"):
+        hip.lu_decomposition(np.ones((2, 3), complex))
+    q = hip.qr_decomposition(np.array([[1, 2, 0], [1, 2, 1], [0, 0, 1]], complex))
+    assert np.allclose(A_(q["Q"])[:, 1], [0, 1, 0]) and abs(A_(q["R"])[1, 1]) < 1e-12
+    assert np.array_equal(A_(hip.lu_decomposition(np.array([[1, 1], [1, 2]], complex))["P"]), np.eye(2))
+    with la.with_backend(hip):
+        assert set(la.lu_decomposition(np.eye(3))) == {"L", "U", "P"}
+        assert set(la.qr_decomposition(np.eye(3))) == {"Q", "R"}
+        assert set(la.cholesky_decomposition(np.eye(3))) == {"L"}

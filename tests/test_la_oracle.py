@@ -170,3 +170,42 @@ def test_matrix_functions_against_scipy(n):
     assert np.allclose(lo.matrix_sqrt(Gp) @ lo.matrix_sqrt(Gp), Gp, atol=1e-10)
     with pytest.raises(lo.OracleLAError):
         lo.matrix_log(np.array([[1.0, 2.0], [0.5, 1.0]]))   # singular
+
+
+@pytest.mark.parametrize("n", [1, 3, 8])
+def test_cholesky_lu_qr_against_lapack(n):
+    from scipy.linalg import lu as sp_lu
+    rng = np.random.default_rng(500 + n)
+    B = rand_c(rng, n, n)
+    P = B @ B.conj().T + n * np.eye(n)
+    assert np.allclose(lo.cholesky(P)["L"], np.linalg.cholesky(P), atol=1e-12)
+    # LU: real input has the reference's factors, the same pivots as LAPACK's getrf
+    R = rng.standard_normal((n, n))
+    f = lo.lu(R)
+    p, l, u = sp_lu(R)
+    assert np.allclose(f["P"], p.T) and np.allclose(f["L"], l, atol=1e-12) and np.allclose(f["U"], u, atol=1e-12)
+    f = lo.lu(B)
+    assert np.allclose(f["P"] @ B, f["L"] @ f["U"], atol=1e-12) and np.all(np.abs(np.tril(f["L"], -1)) <= 1 + 1e-15)
+    # QR: A = QR, orthonormal Q, R upper with real positive diagonal (unique)
+    A = rand_c(rng, n + 2, n)
+    q = lo.qr(A)
+    assert np.allclose(q["Q"] @ q["R"], A, atol=1e-12)
+    assert np.allclose(q["Q"].conj().T @ q["Q"], np.eye(n), atol=1e-12)
+    assert np.allclose(np.tril(q["R"], -1), 0) and np.all(np.diag(q["R"]).real > 0)
+    Qn, Rn = np.linalg.qr(A)
+    ph = np.diag(Rn) / np.abs(np.diag(Rn))
+    assert np.allclose(q["R"], np.diag(ph.conj()) @ Rn, atol=1e-10)
+
+
+def test_decomposition_errors_and_edge_cases():
+    with pytest.raises(lo.OracleLAError, match="Hermitian"):
+        lo.cholesky(np.array([[1, 2], [0, 1]], complex))
+    with pytest.raises(lo.OracleLAError, match="positive definite"):
+        lo.cholesky(np.array([[1, 2], [2, 1]], complex))
+    with pytest.raises(lo.OracleLAError, match="singular"):
+        lo.lu(np.array([[1, 2], [2, 4]], complex))
+    # a dependent column: the reference puts e_j into Q
+    q = lo.qr(np.array([[1, 2, 0], [1, 2, 1], [0, 0, 1]], complex))
+    assert np.allclose(q["Q"][:, 1], [0, 1, 0]) and abs(q["R"][1, 1]) < 1e-12
+    # equal pivot candidates: the first row stays
+    assert np.array_equal(lo.lu(np.array([[1, 1], [1, 2]], complex))["P"], np.eye(2))
