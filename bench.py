@@ -348,6 +348,36 @@ def bench_la(engine, n: int, seed: int) -> dict:
                                          "GEMV on the device, and the dense product on the host (numpy)"}
     D.close()
     psi.close()
+    # decompositions and matrix functions at n = 128: wall clock of one call, and
+    # the residual of the identity each one satisfies
+    k = 128
+    M = rng.standard_normal((k, k)) + 1j * rng.standard_normal((k, k))
+    Hp = M @ M.conj().T / k + np.eye(k)
+    Md = la.DeviceMatrix.from_numpy(M)
+    Hd = la.DeviceMatrix.from_numpy(Hp)
+    dec = {}
+
+    def timed(name, fn, resid):
+        t0 = time.perf_counter()
+        r = fn()
+        ms = (time.perf_counter() - t0) * 1e3
+        dec[name] = {"ms": round(ms, 2), "max_abs_residual": float(resid(r))}
+
+    timed("lu", lambda: hip.lu_decomposition(Md),
+          lambda r: np.max(np.abs(np.asarray(r["P"]) @ M - np.asarray(r["L"]) @ np.asarray(r["U"]))))
+    timed("qr", lambda: hip.qr_decomposition(Md),
+          lambda r: np.max(np.abs(np.asarray(r["Q"]) @ np.asarray(r["R"]) - M)))
+    timed("cholesky", lambda: hip.cholesky_decomposition(Hd),
+          lambda r: np.max(np.abs(np.asarray(r["L"]) @ np.asarray(r["L"]).conj().T - Hp)))
+    timed("svd", lambda: hip.svd(Md),
+          lambda r: np.max(np.abs(np.asarray(r["U"]) @ np.diag([s.real for s in r["S"]]) @ np.asarray(r["V†"]) - M)))
+    timed("matrix_sqrt_hermitian", lambda: hip.matrix_sqrt(Hd),
+          lambda r: np.max(np.abs(np.asarray(r) @ np.asarray(r) - Hp)))
+    timed("matrix_exp_general", lambda: hip.matrix_exp(hip.scale(Md, 1.0 / k)),
+          lambda r: np.max(np.abs(np.asarray(r) @ np.asarray(hip.matrix_exp(hip.scale(Md, -1.0 / k))) - np.eye(k))))
+    out["decompositions_128"] = dec
+    Md.close()
+    Hd.close()
     return out
 
 
