@@ -364,3 +364,21 @@ def test_decomposition_errors(hip):
         assert set(la.lu_decomposition(np.eye(3))) == {"L", "U", "P"}
         assert set(la.qr_decomposition(np.eye(3))) == {"Q", "R"}
         assert set(la.cholesky_decomposition(np.eye(3))) == {"L"}
+
+
+def test_decomposition_edge_cases(hip):
+    note("la: decomposition edge cases")
+    q = hip.qr_decomposition(np.zeros((3, 2), complex))          # every residual is zero: Q gets e_0, e_1
+    assert np.array_equal(A_(q["Q"]), np.eye(3)[:, :2]) and np.array_equal(A_(q["R"]), np.zeros((2, 2)))
+    q = hip.qr_decomposition(np.ones((2, 3), complex))           # n > m: e_2 does not exist, a zero column
+    o = lo.qr(np.ones((2, 3), complex))
+    assert np.allclose(A_(q["Q"]), o["Q"], atol=1e-14) and np.allclose(A_(q["R"]), o["R"], atol=1e-14)
+    f = hip.lu_decomposition([[2.0 - 1j]])
+    assert A_(f["L"])[0, 0] == 1 and A_(f["U"])[0, 0] == 2 - 1j and A_(f["P"])[0, 0] == 1
+    with pytest.raises(la.LinearAlgebraError, match="positive definite"):
+        hip.cholesky_decomposition([[-1.0]])
+    assert A_(hip.cholesky_decomposition([[4.0]])["L"])[0, 0] == 2
+    # a kept Kronecker product goes through the same decompositions
+    K = hip.kronecker_product(np.array([[2.0, 1.0], [1.0, 2.0]]), np.eye(2))
+    L = A_(hip.cholesky_decomposition(K)["L"])
+    assert np.allclose(L @ L.conj().T, np.kron([[2.0, 1.0], [1.0, 2.0]], np.eye(2)), atol=1e-14)
