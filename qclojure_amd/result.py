@@ -96,13 +96,33 @@ def extract_measurement_results(dev: DeviceState, measurement_qubits=None, shots
     }
 
 
+def _state_vector(dev: DeviceState):
+    """The state as a device vector of the :hip math backend: a view of a
+    complex128 state, a complex128 copy of a complex64 one."""
+    from . import la
+    from ._abi import QSV_C128
+    if dev.precision == QSV_C128:
+        return la.DeviceMatrix.view_state(dev)
+    return la.DeviceMatrix.from_numpy(dev.download().astype(np.complex128).reshape(-1, 1))
+
+
+def _dense_expectation(dev: DeviceState, observable) -> complex:
+    """obs/expectation-value (observables.clj:234-251) for an observable of the
+    state's full dimension: <psi| (O psi) with the :hip backend's GEMV and inner
+    product on the device, as the reference computes it through its LA facade."""
+    from . import la
+    hip = la.create_backend("hip")
+    psi = _state_vector(dev)
+    return hip.inner_product(psi, hip.matrix_vector_product(observable, psi))
+
+
 def _embedded_expectation(dev: DeviceState, observable, target):
     obs = np.asarray(observable, dtype=np.complex128)
     if target is None:
         if obs.shape != (2 ** dev.num_qubits,) * 2:
             raise ValueError("observable dimension does not match the state")
         if dev.num_qubits != 1:
-            raise NotImplementedError("untargeted observables are supported for 1-qubit states only")
+            return _dense_expectation(dev, obs)
         target = 0
     return dev.expect_1q(int(target), obs)
 
@@ -124,7 +144,12 @@ def extract_variance_results(dev: DeviceState, observables, target_qubits=None) 
     for o, t in zip(observables, targets):
         m = np.asarray(o, dtype=np.complex128)
         e = _embedded_expectation(dev, m, t).real
-        e2 = _embedded_expectation(dev, m @ m, t).real
+        if t is None and dev.num_qubits != 1 and m.shape == (2 ** dev.num_qubits,) * 2:
+            # O^2 for a full-dimension observable: the device GEMM, then <psi|O^2 psi>
+            from . import la
+            e2 = _dense_expectation(dev, la.create_backend("hip").matrix_multiply(m, m)).real
+        else:
+            e2 = _embedded_expectation(dev, m @ m, t).real
         var = e2 - e * e
         out.append({"variance-value": var, "standard-deviation": float(np.sqrt(var)) if var >= 0 else float("nan"),
                     "observable": o, "target-qubits": [t] if t is not None else None})

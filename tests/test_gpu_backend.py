@@ -172,3 +172,29 @@ def test_sample_result_type(sim):
     c3 = qc.h_gate(qc.x_gate(qc.create_circuit(3), 0), 1)
     r = be.execute_circuit(sim, c3, {"result-specs": {"sample": {"observables": [X], "targets": [1], "shots": 30}}})
     assert r["results"]["sample-results"][0]["frequencies"] == {1.0: 30}
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_full_dimension_observables(oracle, prec):
+    """result.clj:279-281: an observable given without a target is used as-is on
+    the whole state; <psi|O psi> through the :hip GEMV and inner product."""
+    from qclojure_amd import engine
+    from qclojure_amd import result as rs
+    engine.init(0)
+    n = 4
+    ops = wl.random_layer_ops(n, 3, 11)
+    psi = oracle.run(n, ops)
+    rng = np.random.default_rng(3)
+    M = rng.standard_normal((16, 16)) + 1j * rng.standard_normal((16, 16))
+    O = (M + M.conj().T) / 2
+    exp = float(np.real(np.vdot(psi, O @ psi)))
+    var = float(np.real(np.vdot(psi, O @ (O @ psi)))) - exp * exp
+    tol = 1e-11 if prec == "c128" else 1e-5
+    with engine.DeviceState(n, prec) as s:
+        s.apply(ops)
+        e = rs.extract_expectation_results(s, [O])[0]
+        v = rs.extract_variance_results(s, [O])[0]
+        assert abs(e["expectation-value"] - exp) < tol and e["target-qubits"] is None
+        assert abs(v["variance-value"] - var) < 10 * tol
+        with pytest.raises(ValueError):
+            rs.extract_expectation_results(s, [np.eye(8)])
