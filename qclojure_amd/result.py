@@ -248,19 +248,23 @@ def _eigen_probabilities(dev: DeviceState, observable, target) -> list[tuple[flo
     obs = np.asarray(observable, dtype=np.complex128)
     if obs.ndim != 2 or obs.shape[0] != obs.shape[1] or not np.allclose(obs, obs.conj().T, atol=1e-12):
         raise ValueError("observable must be a square Hermitian matrix")
-    w, v = np.linalg.eigh(obs)
     n = dev.num_qubits
     probs = []
     if obs.shape == (2, 2) and (target is not None or n == 1):
+        w, v = np.linalg.eigh(obs)       # a 2x2 on the host, like a gate matrix
         q = 0 if target is None else int(target)
         for i in range(2):
             proj = np.outer(v[:, i], v[:, i].conj())
             probs.append(float(dev.expect_1q(q, proj).real))
     elif obs.shape == (1 << n, 1 << n):
+        # the full-dimension eigensystem by the :hip backend's Jacobi solver
+        from . import la
+        e = la.create_backend("hip").eigen_hermitian(obs)
+        w = np.array([x.real for x in e["eigenvalues"]])
         tmp = DeviceState(n, dev.precision)
         try:
-            for i in range(len(w)):
-                tmp.upload(v[:, i])
+            for vec in e["eigenvectors"]:
+                tmp.upload(vec)
                 probs.append(abs(tmp.inner(dev)) ** 2)
         finally:
             tmp.destroy()
