@@ -30,9 +30,15 @@ READOUT_SALT = 0x52454144_4F555421  # separates the readout stream from the traj
 
 
 def _density_matrix(states: np.ndarray) -> dict:
-    """trajectory-to-density-matrix (state.clj:726-800) with equal weights."""
+    """trajectory-to-density-matrix (state.clj:726-800) with equal weights:
+    rho = (1/k) sum_t psi_t psi_t^H. With the k final states as the rows of T,
+    T^H T = k conj(rho) = k rho^T (rho is Hermitian), so rho is one device GEMM
+    (A^H B, the :hip backend's) and a transpose."""
+    from . import la
     k = len(states)
-    rho = np.einsum("ti,tj->ij", states, states.conj()) / k
+    hip = la.create_backend("hip")
+    T = la.DeviceMatrix.from_numpy(np.ascontiguousarray(states, dtype=np.complex128))
+    rho = hip.transpose(hip.scale(hip._gemm(T, T, conja=1), 1.0 / k)).to_numpy()
     return {"density-matrix": rho, "trace": float(np.real(np.trace(rho))), "weights": [1.0 / k] * k}
 
 
