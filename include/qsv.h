@@ -226,6 +226,7 @@ typedef struct qsv_server_config {
 #define QSV_REQ_RUNNING   1
 #define QSV_REQ_COMPLETED 2
 #define QSV_REQ_FAILED    3
+#define QSV_REQ_CANCELLED 4
 
 typedef struct qsv_request_info {
   int32_t  status;            /* QSV_REQ_*                                          */
@@ -241,6 +242,7 @@ typedef struct qsv_server_stats {
   uint64_t queued_now, inflight_now, max_queue_depth;
   uint64_t small_batches, small_batched_requests;
   uint64_t hbm_budget_bytes, hbm_in_use_bytes;
+  uint64_t cancelled;
 } qsv_server_stats;
 
 int qsv_server_create(const qsv_server_config* cfg, qsv_server** out);
@@ -253,6 +255,15 @@ int qsv_server_status(qsv_server* srv, uint64_t request_id, qsv_request_info* in
  * (if outcomes != NULL) and forgets the request. Returns QSV_E_STATE on timeout. */
 int qsv_server_wait(qsv_server* srv, uint64_t request_id, double timeout_s,
                     qsv_request_info* info, uint64_t* outcomes);
+/* cancel-job (ideal_simulator.clj:154-163): a queued request leaves the queue
+ * at once; an admitted one, whose launches are already on the device, runs to
+ * its end and then finishes as QSV_REQ_CANCELLED without outcomes. *outcome =
+ * QSV_CANCEL_DONE (:cancelled) or QSV_CANCEL_CANNOT (already completed or
+ * failed: :cannot-cancel); unknown ids return QSV_E_NOTFOUND (:not-found).
+ * qsv_server_wait on a cancelled request returns at once with its status. */
+#define QSV_CANCEL_DONE   0
+#define QSV_CANCEL_CANNOT 1
+int qsv_server_cancel(qsv_server* srv, uint64_t request_id, int* outcome);
 int qsv_server_stats_read(qsv_server* srv, qsv_server_stats* out);
 /* Drains in-flight work, then frees every state and the scheduler thread. */
 int qsv_server_destroy(qsv_server* srv);

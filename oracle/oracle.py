@@ -38,6 +38,7 @@ def lib() -> ctypes.CDLL:
             "ref_norm2": (dbl, [vp, i32]),
             "ref_measure_state": (u64, [vp, i32, dbl]),
             "ref_sample": (None, [vp, i32, u64, u64, vp]),
+            "ref_sample_prefix": (None, [vp, i32, u64, u64, vp, vp]),
             "ref_measure_qubits": (i32, [vp, i32, u64, dbl, vp, vp]),
             "ref_pauli_expectation": (dbl, [vp, i32, ctypes.c_char_p]),
             "ref_pauli_expectation_literal": (dbl, [vp, i32, ctypes.c_char_p]),
@@ -97,9 +98,29 @@ def norm2(psi: np.ndarray, n: int) -> float:
     return lib().ref_norm2(_p(np.ascontiguousarray(psi)), n)
 
 
+def normalize(psi: np.ndarray, n: int, tolerance: float = 1e-12) -> np.ndarray:
+    """normalize-state (state.clj:524-551): divide by norm = sqrt(sum |a|^2)
+    when norm > 0 and norm > the LA tolerance (1e-12 by default,
+    fastmath/backend.clj:23), else return the amplitudes unchanged. The scale
+    is 1/norm applied as a multiplication (fc/scale a (/ 1.0 norm))."""
+    a = np.ascontiguousarray(psi, dtype=np.complex128)
+    nrm = float(np.sqrt(norm2(a, n)))
+    if nrm > 0 and nrm > tolerance:
+        return a * (1.0 / nrm)
+    return a.copy()
+
+
 def sample(psi: np.ndarray, n: int, shots: int, seed: int) -> np.ndarray:
     out = np.zeros(shots, dtype=np.uint64)
     lib().ref_sample(_p(np.ascontiguousarray(psi)), n, shots, seed, _p(out))
+    return out
+
+
+def sample_prefix(psi: np.ndarray, n: int, shots: int, seed: int) -> np.ndarray:
+    """ref_sample's shots with the running sums formed once (large states)."""
+    out = np.zeros(shots, dtype=np.uint64)
+    prefix = np.empty(1 << n)
+    lib().ref_sample_prefix(_p(np.ascontiguousarray(psi)), n, shots, seed, _p(prefix), _p(out))
     return out
 
 
@@ -169,9 +190,7 @@ def trajectories(n: int, ops: np.ndarray, sites, n_traj: int, seed: int, make_u)
                         if u < acc or j == len(kr) - 1:
                             break
                 psi = apply_program(psi, n, make_u(sites[k]["qubit"], np.asarray(kr[j], dtype=np.complex128)))
-                nrm = np.sqrt(norm2(psi, n))
-                if nrm > 1e-10:
-                    psi = psi / nrm
+                psi = normalize(psi, n)
                 k += 1
         seed_m = (seed + (t * TRAJ_SLOTS + TRAJ_SLOTS - 1) * _PHI) % (1 << 64)
         outs[t] = sample(psi, n, 1, seed_m)[0]

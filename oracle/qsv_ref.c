@@ -454,6 +454,31 @@ void ref_sample(const double* amps, int n, uint64_t shots, uint64_t seed, uint64
   for (uint64_t k = 0; k < shots; ++k) out[k] = ref_measure_state(amps, n, ref_uniform(seed, k));
 }
 
+/* The same shots as ref_sample for large states: the running sums of
+ * measure-state (state.clj:869-913) are formed once, in the same order, into
+ * `prefix` (2^n doubles, caller-provided), and each shot finds the first
+ * running sum >= u * total by binary search, which is the count of running
+ * sums below it because the sums never decrease. */
+void ref_sample_prefix(const double* amps, int n, uint64_t shots, uint64_t seed, double* prefix, uint64_t* out)
+{
+  const uint64_t S = 1ull << n;
+  double cum = 0;
+  for (uint64_t i = 0; i < S; ++i) {
+    cum += amps[2 * i] * amps[2 * i] + amps[2 * i + 1] * amps[2 * i + 1];
+    prefix[i] = cum;
+  }
+  const double total = ref_norm2(amps, n);
+  for (uint64_t k = 0; k < shots; ++k) {
+    const double r = ref_uniform(seed, k) * total;
+    uint64_t lo = 0, hi = S;   /* first index with prefix >= r, S if none */
+    while (lo < hi) {
+      const uint64_t mid = lo + (hi - lo) / 2;
+      if (prefix[mid] < r) lo = mid + 1; else hi = mid;
+    }
+    out[k] = lo < S ? lo : S - 1;
+  }
+}
+
 /* pauli-string expectation by the Kronecker definition (observables.clj:216-251):
  * (P psi)_i = sum_j P_ij psi_j with P = sigma_{s[0]} ⊗ ... ⊗ sigma_{s[n-1]},
  * evaluated per row (each row has a single non-zero). */

@@ -110,10 +110,11 @@ class QsvRequestInfo(ctypes.Structure):
 class QsvServerStats(ctypes.Structure):
     _fields_ = [(k, ctypes.c_uint64) for k in (
         "submitted", "completed", "failed", "queued_now", "inflight_now", "max_queue_depth",
-        "small_batches", "small_batched_requests", "hbm_budget_bytes", "hbm_in_use_bytes")]
+        "small_batches", "small_batched_requests", "hbm_budget_bytes", "hbm_in_use_bytes", "cancelled")]
 
 
-REQ_STATUS = {0: "queued", 1: "running", 2: "completed", 3: "failed"}
+REQ_STATUS = {0: "queued", 1: "running", 2: "completed", 3: "failed", 4: "cancelled"}
+CANCEL_OUTCOME = {0: "cancelled", 1: "cannot-cancel"}
 
 # Every symbol include/qsv.h declares, with (restype, argtypes).
 _vp, _u64, _i32, _u32, _dbl = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint32, ctypes.c_double
@@ -150,6 +151,7 @@ SIGNATURES = {
     "qsv_server_submit": (_i32, [_vp, _i32, _vp, _u64, _u64, _u64, _pu64]),
     "qsv_server_status": (_i32, [_vp, _u64, ctypes.POINTER(QsvRequestInfo)]),
     "qsv_server_wait": (_i32, [_vp, _u64, _dbl, ctypes.POINTER(QsvRequestInfo), _vp]),
+    "qsv_server_cancel": (_i32, [_vp, _u64, ctypes.POINTER(ctypes.c_int)]),
     "qsv_server_stats_read": (_i32, [_vp, ctypes.POINTER(QsvServerStats)]),
     "qsv_server_destroy": (_i32, [_vp]),
     "qsv_timing_enable": (_i32, [_vp, _i32]),

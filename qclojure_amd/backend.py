@@ -24,7 +24,7 @@ import numpy as np
 from . import _abi
 from .circuit import CircuitError, circuit_metadata, decode_circuit, NATIVE_SIMULATOR_GATE_SET
 from .engine import DeviceState, device_info
-from .result import FinalState, extract_results
+from .result import FinalState, detach_results, extract_results, release_results
 
 
 class QuantumBackend(abc.ABC):
@@ -117,8 +117,10 @@ class GpuSimulator(QuantumBackend, BatchJobBackend):
     "c64"), "seed" (base seed for shots and mid-circuit measurement; each job
     uses seed + job counter), "workers" (concurrent jobs, default 4),
     "msb-swap" (True = SWAP/iSWAP on MSB bit positions instead of the
-    reference's LSB positions), "keep-final-state" (keep the device state alive
-    in the result, default True).
+    reference's LSB positions), "keep-final-state" (default True: the result's
+    :final-state and lazily read vectors stay in HBM until read or until
+    cleanup_completed_jobs drops the job; False: they are copied to host memory
+    when the job completes and the device state is freed at once).
     """
 
     def __init__(self, config: dict | None = None):
@@ -196,6 +198,8 @@ class GpuSimulator(QuantumBackend, BatchJobBackend):
                     "circuit-metadata": circuit_metadata(circuit)}
             results = extract_results(base, specs, dev, seed ^ 0xA5A5A5A5A5A5A5A5) if specs else base
             dev.sync()
+            if not self.config.get("keep-final-state", True):
+                detach_results(results)
             return {"job-status": "completed", "results": results,
                     "execution-time-ms": (time.perf_counter() - start) * 1000.0}
         except Exception as e:  # every failure becomes a :failed job, never a hang
@@ -271,6 +275,22 @@ class GpuSimulator(QuantumBackend, BatchJobBackend):
 
 def create_simulator(config: dict | None = None) -> GpuSimulator:
     return GpuSimulator(config or {})
+
+
+def cleanup_completed_jobs(max_age_ms: int = 60 * 60 * 1000) -> int:
+    """cleanup-completed-jobs! (ideal_simulator.clj:243-263): drop completed
+    jobs that completed more than max_age_ms ago and return how many were
+    dropped. The device memory of a dropped job's final state is freed."""
+    cutoff = _now_ms() - max_age_ms
+    with _STATE_LOCK:
+        jobs = _SIM_STATE["active-jobs"]
+        drop = [jid for jid, j in jobs.items()
+                if j["status"] == "completed" and (j["completed-at"] or 0) <= cutoff]
+        dropped = [jobs.pop(jid) for jid in drop]
+    for j in dropped:
+        if j.get("result"):
+            release_results(j["result"].get("results"))
+    return len(dropped)
 
 
 def get_simulator_stats() -> dict:

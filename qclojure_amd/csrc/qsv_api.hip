@@ -325,6 +325,8 @@ bool qok(int q, int n) { return q >= 0 && q < n; }
 
 // A decoded gate in global index-bit coordinates (bit b = n-1-q).
 enum { BG_MEASURE = 10 };
+// Records that act on the whole state and never enter a fused pass.
+inline bool runs_alone(int kind) { return kind == BG_MEASURE || kind == FG_RENORM; }
 
 struct BGate {
   int kind = FG_DENSE;     // FG_DENSE, FG_PHASE, FG_SWAP, FG_BLOCKADE or BG_MEASURE
@@ -496,6 +498,9 @@ int run_direct(qsv_state* s, const BGate& g)
   case FG_BLOCKADE:
     return timed(s, QSV_KCLASS_DIAG, 2.0 * s->S * s->bytes_per_amp(),
                  [&] { launch_blockade(s->prec, s->amps, s->S, g.bmask, g.m, s->stream); });
+  case FG_RENORM:
+    return timed(s, QSV_KCLASS_OTHER, 3.0 * s->S * s->bytes_per_amp(),
+                 [&] { launch_renorm(s->prec, s->amps, s->S, s->partials, s->dout, s->stream); });
   default:
     return measure_impl(s, g.bmask, g.u, nullptr, nullptr);
   }
@@ -707,7 +712,7 @@ int fuse_walk(const qsv_state* s, const std::vector<BGate>& gs, bool f32, Emit&&
     while (first < N && done[first]) ++first;
     if (first >= N) break;
     const BGate& g0 = gs[first];
-    if (g0.kind == BG_MEASURE || __builtin_popcountll(gbits[first] | lowmask) > TB) {
+    if (runs_alone(g0.kind) || __builtin_popcountll(gbits[first] | lowmask) > TB) {
       if ((rc = barrier(first))) return rc;
       done[first] = 1;
       run_first = true;
@@ -721,7 +726,7 @@ int fuse_walk(const qsv_state* s, const std::vector<BGate>& gs, bool f32, Emit&&
     win.clear();
     for (size_t i = first; i < N && win.size() < kWindow; ++i) {
       if (done[i]) continue;
-      if (gs[i].kind == BG_MEASURE) break;
+      if (runs_alone(gs[i].kind)) break;
       win.push_back(i);
     }
     uint64_t set = lowmask | gbits[first];
@@ -751,7 +756,7 @@ int fuse_walk(const qsv_state* s, const std::vector<BGate>& gs, bool f32, Emit&&
       if (done[i]) continue;
       ++scanned;
       const BGate& g = gs[i];
-      if (g.kind == BG_MEASURE) break;
+      if (runs_alone(g.kind)) break;
       const uint64_t gb = gbits[i];
       if (!(gb & blocked)) {
         const uint64_t nu = set | gb;
@@ -1785,8 +1790,14 @@ int qsv_noisy_trajectories(int num_qubits, int precision, const qsv_op* ops, uin
     }
     return nk - 1;
   };
-  // one trajectory's gate list: clean gates of op i, then its sites' operators
-  auto trajectory_gates = [&](uint64_t t, std::vector<BGate>& out) {
+  // one trajectory's gate list: clean gates of op i, then its sites' operators.
+  // A chosen operator scaled to max |k_ij| = 1 shrinks the norm (damping
+  // operators do, site after site), so a renormalisation record follows every
+  // `renorm_every`-th site: every site in the LDS batch path, where it is one
+  // sweep of LDS; every 16th between fused passes, where it is a barrier of
+  // two state passes. Without it, deep programs with strong damping would
+  // drive the amplitudes into the subnormal range.
+  auto trajectory_gates = [&](uint64_t t, std::vector<BGate>& out, uint64_t renorm_every) {
     out.clear();
     size_t g = 0, k = 0;
     for (uint64_t i = 0; i < n_ops; ++i) {
@@ -1794,6 +1805,11 @@ int qsv_noisy_trajectories(int num_qubits, int precision, const qsv_op* ops, uin
       for (; k < n_sites && sites[k].after_op == i; ++k) {
         const int j = choose(k, qsv_uniform(seed, t * QSV_TRAJ_SLOTS + k));
         emit_ctl(out, {}, n - 1 - sites[k].qubit, kops[k][j].data());
+        if ((k + 1) % renorm_every == 0) {
+          BGate r;
+          r.kind = FG_RENORM;
+          out.push_back(std::move(r));
+        }
       }
     }
   };
@@ -1865,7 +1881,7 @@ int qsv_noisy_trajectories(int num_qubits, int precision, const qsv_op* ops, uin
       std::vector<int32_t> order(c);
       std::vector<SampleJob> jobs(c);
       for (uint64_t i = 0; i < c; ++i) {
-        trajectory_gates(t0 + i, gs);
+        trajectory_gates(t0 + i, gs, 1);
         BInst& in = insts[i];
         in.n = n;
         in.g0 = recs.size();
@@ -1928,7 +1944,7 @@ int qsv_noisy_trajectories(int num_qubits, int precision, const qsv_op* ops, uin
     return fail(QSV_E_NOMEM, "trajectories: allocation failed");
   }
   for (uint64_t t = 0; t < n_traj && !rc; ++t) {
-    trajectory_gates(t, gs);
+    trajectory_gates(t, gs, 16);
     launch_fill_basis(s->prec, s->amps, s->S, 0, s->stream);
     if ((rc = run_fused(s, gs, 0))) break;
     launch_sample(s->prec, s->amps, s->S, s->prefix, 1, measure_seed(t), d_out, s->stream);
@@ -1965,6 +1981,7 @@ struct SrvReq {
   std::vector<uint64_t> outcomes;
   uint64_t bytes = 0;
   bool small = false;
+  bool cancel = false;      // cancelled while admitted: finishes as QSV_REQ_CANCELLED
   // in flight (large requests)
   qsv_state* st = nullptr;
   uint64_t* d_out = nullptr;
@@ -2074,13 +2091,31 @@ void give_state(qsv_server* sv, qsv_state* s)
 
 void finish(qsv_server* sv, SrvReq* r, int status, const std::string& err)
 {
-  // called with sv->mu held
+  // called with sv->mu held, for an admitted request
+  if (r->cancel && status == QSV_REQ_COMPLETED) {
+    status = QSV_REQ_CANCELLED;
+    r->outcomes.clear();
+  }
   r->status = status;
   r->err = err;
   r->t_done = Clock::now();
   sv->in_use -= r->bytes;
   if (status == QSV_REQ_COMPLETED) sv->stats.completed++;
+  else if (status == QSV_REQ_CANCELLED) sv->stats.cancelled++;
   else sv->stats.failed++;
+}
+
+// Outcome of polling a completion event: hipErrorNotReady means the work is
+// still in flight; any status other than that and hipSuccess is a device
+// error, which fails the request instead of leaving it running for ever.
+enum { POLL_DONE, POLL_PENDING, POLL_FAILED };
+int poll_event(hipEvent_t ev, std::string* err)
+{
+  const hipError_t e = hipEventQuery(ev);
+  if (e == hipSuccess) return POLL_DONE;
+  if (e == hipErrorNotReady) return POLL_PENDING;
+  *err = std::string("device error while the request ran: ") + hipGetErrorString(e);
+  return POLL_FAILED;
 }
 
 // Launch every pending small request as one batched launch on a free slot.
@@ -2199,14 +2234,18 @@ int launch_large(qsv_server* sv, SrvReq* r)
   return QSV_OK;
 }
 
-void release_large(qsv_server* sv, SrvReq* r)
+void release_large(qsv_server* sv, SrvReq* r, bool device_error = false)
 {
   // the shot buffers belong to the state, which goes back to the cache
+  // (not after a device error: then it is freed)
   if (r->ev) hipEventDestroy(r->ev);
   r->ev = nullptr;
   r->d_out = nullptr;
   r->h_out = nullptr;
-  if (r->st) give_state(sv, r->st);
+  if (r->st) {
+    if (device_error) qsv_state_destroy(r->st);
+    else give_state(sv, r->st);
+  }
   r->st = nullptr;
 }
 
@@ -2220,6 +2259,7 @@ void server_loop(qsv_server* sv)
     const int cap = sv->cfg.max_inflight > 0 ? sv->cfg.max_inflight : 32;
     while (!sv->queue.empty()) {
       SrvReq* r = sv->queue.front();
+      // (cancelled requests leave the queue in qsv_server_cancel)
       if (sv->in_use + r->bytes > sv->budget && sv->in_use > 0) break;
       if (sv->inflight() >= cap) break;
       sv->queue.pop_front();
@@ -2259,14 +2299,16 @@ void server_loop(qsv_server* sv)
       if (!failed.empty()) sv->cv_done.notify_all();
       progressed = true;
     }
-    // 3. completions
+    // 3. completions (a device error fails the request and frees what it held)
     bool done_any = false;
+    std::string perr;
     for (size_t k = 0; k < sv->running_large.size();) {
       SrvReq* r = sv->running_large[k];
-      if (hipEventQuery(r->ev) == hipSuccess) {
-        r->outcomes.assign(r->h_out, r->h_out + r->shots);
-        release_large(sv, r);
-        finish(sv, r, QSV_REQ_COMPLETED, "");
+      const int pr = poll_event(r->ev, &perr);
+      if (pr != POLL_PENDING) {
+        if (pr == POLL_DONE) r->outcomes.assign(r->h_out, r->h_out + r->shots);
+        release_large(sv, r, pr == POLL_FAILED);
+        finish(sv, r, pr == POLL_DONE ? QSV_REQ_COMPLETED : QSV_REQ_FAILED, pr == POLL_DONE ? "" : perr);
         sv->running_large[k] = sv->running_large.back();
         sv->running_large.pop_back();
         done_any = true;
@@ -2275,16 +2317,21 @@ void server_loop(qsv_server* sv)
       }
     }
     for (auto& slot : sv->slots) {
-      if (slot.busy && hipEventQuery(slot.ev) == hipSuccess) {
-        for (size_t k = 0; k < slot.reqs.size(); ++k) {
-          SrvReq* r = slot.reqs[k];
+      if (!slot.busy) continue;
+      const int pr = poll_event(slot.ev, &perr);
+      if (pr == POLL_PENDING) continue;
+      for (size_t k = 0; k < slot.reqs.size(); ++k) {
+        SrvReq* r = slot.reqs[k];
+        if (pr == POLL_DONE) {
           r->outcomes.assign(slot.h_out + slot.out_offsets[k], slot.h_out + slot.out_offsets[k] + r->shots);
           finish(sv, r, QSV_REQ_COMPLETED, "");
+        } else {
+          finish(sv, r, QSV_REQ_FAILED, perr);
         }
-        slot.reqs.clear();
-        slot.busy = false;
-        done_any = true;
       }
+      slot.reqs.clear();
+      slot.busy = false;
+      done_any = true;
     }
     if (done_any) {
       sv->cv_done.notify_all();
@@ -2413,6 +2460,36 @@ int qsv_server_wait(qsv_server* sv, uint64_t request_id, double timeout_s, qsv_r
   if (outcomes && r->status == QSV_REQ_COMPLETED) std::memcpy(outcomes, r->outcomes.data(), r->shots * sizeof(uint64_t));
   sv->reqs.erase(it);
   delete r;
+  return QSV_OK;
+}
+
+int qsv_server_cancel(qsv_server* sv, uint64_t request_id, int* outcome)
+{
+  if (!sv || !outcome) return fail(QSV_E_INVALID, "null argument");
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    auto it = sv->reqs.find(request_id);
+    if (it == sv->reqs.end()) return fail(QSV_E_NOTFOUND, "unknown request id");
+    SrvReq* r = it->second;
+    if (r->status >= QSV_REQ_COMPLETED) {
+      *outcome = QSV_CANCEL_CANNOT;
+      return QSV_OK;
+    }
+    *outcome = QSV_CANCEL_DONE;
+    if (r->status == QSV_REQ_QUEUED) {
+      // never admitted: it holds no device memory and leaves the queue now
+      auto q = std::find(sv->queue.begin(), sv->queue.end(), r);
+      if (q != sv->queue.end()) sv->queue.erase(q);
+      r->t_admit = r->t_done = Clock::now();
+      r->status = QSV_REQ_CANCELLED;
+      sv->stats.cancelled++;
+    } else {
+      // admitted: its launches are on the device; it finishes as cancelled
+      r->cancel = true;
+      return QSV_OK;
+    }
+  }
+  sv->cv_done.notify_all();
   return QSV_OK;
 }
 

@@ -86,9 +86,26 @@ __global__ __launch_bounds__(kBatchThreads) void k_lds_batch(const BInst* __rest
         st[base | ba] = bmul<T, R>(g.m[0], g.m[1], b);
         st[base | bb] = bmul<T, R>(g.m[0], g.m[1], a);
       }
-    } else {  // FG_BLOCKADE
+    } else if (g.kind == FG_BLOCKADE) {
       for (uint64_t j = tid; j < S; j += nt)
         if (__popcll(j & g.bmask) == 1) st[j] = bmul<T, R>(g.m[0], g.m[1], st[j]);
+    } else {  // FG_RENORM: normalize-state after a Kraus operator, FP64 norm
+      double acc = 0.0;
+      for (uint64_t j = tid; j < S; j += nt) {
+        const T a = st[j];
+        acc += (double)a.x * a.x + (double)a.y * a.y;
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+      if ((tid & 63) == 0) red[tid >> 6] = acc;
+      __syncthreads();
+      double tot = 0.0;
+      for (uint32_t w = 0; w < nt / 64; ++w) tot += red[w];   // same order in every thread
+      const double nrm = sqrt(tot);
+      if (nrm > 1e-12) {
+        const double inv = 1.0 / nrm;
+        for (uint64_t j = tid; j < S; j += nt) st[j] = bmul<T, R>(inv, 0.0, st[j]);
+      }
     }
     __syncthreads();
   }

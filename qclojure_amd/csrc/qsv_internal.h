@@ -66,6 +66,9 @@ void launch_blockade(int prec, void* psi, uint64_t S, uint64_t bmask, const doub
 // qsv_kernels.hip
 void launch_fill_basis(int prec, void* psi, uint64_t S, uint64_t idx, hipStream_t st);
 void launch_scale(int prec, void* psi, uint64_t S, double f, hipStream_t st);
+// normalize-state without a host round trip: FP64 norm into out[0], then
+// the scale by 1/norm when the norm exceeds 1e-12 (partials: kReduceBlocks doubles)
+void launch_renorm(int prec, void* psi, uint64_t S, double* partials, double* out, hipStream_t st);
 void launch_copy(int prec, const void* src, void* dst, uint64_t S, hipStream_t st);
 // out[0..4] = Re <a|b>, Im <a|b>, sum |a|^2, sum |b|^2, max |a_i - b_i|;
 // partials holds 5 * kReduceBlocks doubles.
@@ -102,7 +105,10 @@ constexpr int kTileBitsC64 = 12;    // 4096 x  8 B = 32 KiB: three workgroups pe
 constexpr int kMaxPassGates = 96;
 constexpr int kMaxPassStages = 32;
 
-enum { FG_DENSE = 0, FG_PHASE = 1, FG_SWAP = 2, FG_BLOCKADE = 3 };
+// FG_RENORM: divide the state by its FP64 norm when that exceeds 1e-12, as
+// normalize-state does after a Kraus operator (a record of batched LDS
+// circuits, a barrier between fused passes; never inside a pass).
+enum { FG_DENSE = 0, FG_PHASE = 1, FG_SWAP = 2, FG_BLOCKADE = 3, FG_RENORM = 4 };
 // Structure of a dense 2x2, which decides the arithmetic it needs per pair:
 // general (16 real multiply-adds), real entries (8), real diagonal with
 // imaginary off-diagonal, as RX (8), or the exchange [[0,1],[1,0]] (none).

@@ -108,6 +108,24 @@ __global__ __launch_bounds__(kBlock) void k_scale(typename VT<R>::T* __restrict_
   }
 }
 
+// normalize-state with the norm^2 already on the device (norm2[0]): divide by
+// the norm when it exceeds 1e-12, else leave the amplitudes.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_scale_by_norm(typename VT<R>::T* __restrict__ psi, uint64_t S,
+                                                          const double* __restrict__ norm2)
+{
+  const double nrm = sqrt(norm2[0]);
+  if (!(nrm > 1e-12)) return;
+  const R f = (R)(1.0 / nrm);
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < S; i += stride) {
+    auto v = psi[i];
+    v.x *= f;
+    v.y *= f;
+    psi[i] = v;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_f64_to_c64(const double2* __restrict__ src, float2* __restrict__ dst,
                                                         uint64_t count)
 {
@@ -514,6 +532,16 @@ void launch_scale(int prec, void* psi, uint64_t S, double f, hipStream_t st)
     hipLaunchKernelGGL((k_scale<double>), dim3(g), dim3(kBlock), 0, st, (double2*)psi, S, f);
   else
     hipLaunchKernelGGL((k_scale<float>), dim3(g), dim3(kBlock), 0, st, (float2*)psi, S, (float)f);
+}
+
+void launch_renorm(int prec, void* psi, uint64_t S, double* partials, double* out, hipStream_t st)
+{
+  launch_norm2(prec, psi, S, partials, out, st);
+  const unsigned g = stream_grid(S);
+  if (prec == QSV_C128)
+    hipLaunchKernelGGL((k_scale_by_norm<double>), dim3(g), dim3(kBlock), 0, st, (double2*)psi, S, out);
+  else
+    hipLaunchKernelGGL((k_scale_by_norm<float>), dim3(g), dim3(kBlock), 0, st, (float2*)psi, S, out);
 }
 
 void launch_f64_to_c64(const double* src, void* dst, uint64_t count, hipStream_t st)

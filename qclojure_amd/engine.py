@@ -304,6 +304,15 @@ class Server:
         check(lib().qsv_server_wait(self._h, rid, float(timeout_s), ctypes.byref(info), _ptr(out)))
         return self._info(info), out[:int(shots)]
 
+    def cancel(self, rid: int) -> str:
+        """cancel-job: "cancelled", "cannot-cancel" or "not-found"."""
+        out = ctypes.c_int()
+        rc = lib().qsv_server_cancel(self._h, rid, ctypes.byref(out))
+        if rc == _abi.QSV_E_NOTFOUND:
+            return "not-found"
+        check(rc)
+        return _abi.CANCEL_OUTCOME[out.value]
+
     def stats(self) -> dict:
         st = _abi.QsvServerStats()
         check(lib().qsv_server_stats_read(self._h, ctypes.byref(st)))

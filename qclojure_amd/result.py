@@ -75,6 +75,47 @@ class FinalState:
     def __contains__(self, key):
         return key in ("state-vector", "num-qubits")
 
+    @property
+    def on_device(self) -> bool:
+        return self.device_state is not None
+
+    def detach(self) -> None:
+        """Copy the amplitudes to host memory and free the device state."""
+        if self.device_state is not None:
+            self._sv.materialize()
+            self.device_state.destroy()
+            self.device_state = None
+
+    def release(self) -> None:
+        """Free the device state without keeping the amplitudes (a job table
+        dropping the result)."""
+        if self.device_state is not None:
+            self.device_state.destroy()
+            self.device_state = None
+
+
+def detach_results(results: dict) -> None:
+    """Materialise every lazily computed vector of a result map into host
+    memory, then free the final state's device memory, so that the map no
+    longer holds HBM."""
+    def walk(v):
+        if isinstance(v, LazyVector):
+            v.materialize()
+        elif isinstance(v, dict):
+            for x in v.values():
+                walk(x)
+    walk(results)
+    fs = results.get("final-state")
+    if isinstance(fs, FinalState):
+        fs.detach()
+
+
+def release_results(results: dict) -> None:
+    """Free the device memory a result map holds (its final state)."""
+    fs = results.get("final-state") if isinstance(results, dict) else None
+    if isinstance(fs, FinalState):
+        fs.release()
+
 
 def basis_label(i: int, n: int) -> str:
     return "|" + format(i, f"0{n}b") + "⟩"

@@ -200,3 +200,26 @@ def serving_stream(num_requests: int, rate_hz: float, seed: int, n_min: int = 10
         out.append({"arrival-s": t, "n": n, "depth": depth, "seed": int(rng.integers(0, 2**31)),
                     "shots": shots})
     return out
+
+
+_SELF_INVERSE = ("X", "Y", "Z", "H", "CNOT", "CZ", "SWAP", "TOFFOLI", "FREDKIN", "RYDBERG_CZ")
+_NEGATE = ("RX", "RY", "RZ", "PHASE", "CRX", "CRY", "CRZ", "RYDBERG_CPHASE", "RYDBERG_BLOCKADE",
+           "GLOBAL_RX", "GLOBAL_RY", "GLOBAL_RZ")
+
+
+def inverse_ops(ops: np.ndarray) -> np.ndarray:
+    """U^-1 of a program of self-inverse gates and rotations: the ops reversed,
+    rotation and phase angles negated. Raises for kinds outside those sets. The
+    reference's inverse-gate (circuit.clj:1128-1173) is not followed: it
+    negates only RX/RY/RZ and returns controlled rotations and phase gates
+    unchanged."""
+    allowed_self = {OP[k] for k in _SELF_INVERSE}
+    allowed_neg = {OP[k] for k in _NEGATE}
+    out = ops[::-1].copy()
+    for r in out:
+        k = int(r["kind"])
+        if k in allowed_neg:
+            r["angle"] = -r["angle"]
+        elif k not in allowed_self:
+            raise ValueError(f"no inverse rule for op kind {k}")
+    return out

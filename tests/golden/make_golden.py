@@ -224,6 +224,25 @@ MAXCUT = [
      "coefficient-set": [1.0, -1.0]},
 ]
 
+# state_test.clj:145-166 probability (qs/probability state i = |a_i|^2, state.clj:627-656):
+# "exact" where the reference compares with =. state_test.clj:112-139 normalize-state
+# (state.clj:524-551): {3, 4} -> norm^2 1 within 1e-10 (the docstring's [0.6, 0.8]),
+# |+> unchanged within 1e-10.
+ST = "test/org/soulspace/qclojure/domain/state_test.clj"
+PROBS = [
+    {"id": "p|0>", "source": ST + ":146-149", "n": 1, "init": basis(1, 0), "probs": [1.0, 0.0], "tol": "exact"},
+    {"id": "p|1>", "source": ST + ":151-154", "n": 1, "init": basis(1, 1), "probs": [0.0, 1.0], "tol": "exact"},
+    {"id": "p|+>", "source": ST + ":156-159", "n": 1, "init": PLUS, "probs": [0.5, 0.5], "tol": 1e-10},
+    {"id": "p|00>", "source": ST + ":161-166", "n": 2, "init": basis(2, 0), "probs": [1.0, 0.0, 0.0, 0.0],
+     "tol": "exact"},
+]
+NORMALIZE = [
+    {"id": "norm{3,4}", "source": ST + ":115-122 (value from state.clj:540-541)", "n": 1,
+     "init": [[3.0, 0.0], [4.0, 0.0]], "expect": [[0.6, 0.0], [0.8, 0.0]], "norm2-tol": 1e-10, "tol": 1e-10},
+    {"id": "norm|+>", "source": ST + ":124-139", "n": 1, "init": PLUS, "expect": PLUS, "norm2-tol": 1e-10,
+     "tol": 1e-10},
+]
+
 # state_test.clj:197-226 bits-to-index
 BITS = [([0], 0), ([1], 1), ([0, 0], 0), ([0, 1], 1), ([1, 0], 2), ([1, 1], 3),
         ([0, 0, 0], 0), ([0, 0, 1], 1), ([0, 1, 0], 2), ([0, 1, 1], 3), ([1, 0, 0], 4),
@@ -238,6 +257,8 @@ def main():
                               for (i, s, n, st, h, v, t) in EXPECT],
         "shot-cases": SHOTS,
         "maxcut-cases": MAXCUT,
+        "probability-cases": PROBS,
+        "normalize-cases": NORMALIZE,
         "bits-to-index": [{"bits": b, "index": i, "source": "test/org/soulspace/qclojure/domain/state_test.clj:197-226"}
                           for b, i in BITS],
     }

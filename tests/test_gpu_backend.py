@@ -104,6 +104,36 @@ def test_failed_jobs_never_hang(sim):
     assert sim.job_result(job)["exception-type"] == "CircuitError"
 
 
+def test_cleanup_and_keep_final_state_release_hbm(sim):
+    """Completed jobs hold their final states in HBM until cleanup-completed-jobs
+    (ideal_simulator.clj:243-263) drops them; with keep-final-state false a job
+    copies its results to host memory and frees the device state at once."""
+    from qclojure_amd.engine import device_info
+    n, jobs = 27, 4                       # 2 GiB per complex128 state
+    circ = qc.ghz_state_circuit(n)
+    base = device_info()["hbm-free-bytes"]
+    ids = [sim.submit_circuit(circ, {"result-specs": {"probabilities": {"targets": [0]}}}) for _ in range(jobs)]
+    for j in ids:
+        assert be.wait_for_job(sim, j, 60000)["job-status"] == "completed"
+    held = base - device_info()["hbm-free-bytes"]
+    assert held >= jobs * (16 << n) * 0.99, held
+    assert be.cleanup_completed_jobs(0) == jobs
+    back = base - device_info()["hbm-free-bytes"]
+    assert back < (16 << n) * 0.5, back
+    assert sim.job_status(ids[0]) == "not-found"
+    host = be.create_simulator({"seed": 1, "keep-final-state": False})
+    try:
+        r = be.execute_circuit(host, circ, {"result-specs": {"measurements": {"shots": 10}}})
+        assert r["job-status"] == "completed"
+        assert base - device_info()["hbm-free-bytes"] < (16 << n) * 0.5
+        sv = np.asarray(r["results"]["final-state"]["state-vector"])
+        assert abs(sv[0] - 2 ** -0.5) < 1e-12 and abs(sv[-1] - 2 ** -0.5) < 1e-12
+        p = np.asarray(r["results"]["measurement-results"]["measurement-probabilities"])
+        assert abs(p[0] - 0.5) < 1e-12 and set(r["results"]["measurement-results"]["frequencies"]) <= {0, (1 << n) - 1}
+    finally:
+        host.close()
+
+
 def test_not_found_and_cancel(sim):
     r = sim.job_result("non-existent-id")
     assert r["job-status"] == "not-found" and r["job-id"] == "non-existent-id" and "error-message" in r

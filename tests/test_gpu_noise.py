@@ -73,6 +73,35 @@ def test_trajectories_c64(eng, oracle):
     assert np.min(ov) > 1 - 1e-6
 
 
+@pytest.mark.parametrize("prec", ["c64", "c128"])
+@pytest.mark.parametrize("n", [4, 15])
+def test_deep_damping_stays_normalised(eng, oracle, prec, n):
+    """800 amplitude-damping sites (gamma = 0.5), each after an H: the chosen
+    operator K0 shrinks the norm^2 by about 3/4 per site, 1e-100 over the
+    program, which underflows complex64 (and nears FP64's range in the large
+    path) unless the state is renormalised as the reference does after every
+    Kraus operator (channel.clj:148-199). Outcomes and final states against
+    the oracle, in the LDS batch path (n = 4) and the fused path (n = 15)."""
+    sites = 800 if n == 4 else 200
+    c = qc.create_circuit(n)
+    for k in range(sites):
+        c = qc.h_gate(c, k % n)
+    model = {"gate-noise": {"h": {"noise-type": "amplitude-damping", "noise-strength": 0.5}}}
+    ops, st = program_with_noise(c, model)
+    T = 16 if n == 4 else 2
+    note(f"deep damping {prec} n={n} sites={len(st)} T={T}")
+    got, states = eng.noisy_trajectories(n, ops, st, T, 77, prec, n_keep=T)
+    exp, fin = oracle.trajectories(n, ops, st, T, 77, qc.op_u)
+    assert np.all(np.isfinite(states))
+    ov = np.abs(np.sum(states.conj() * fin, axis=1)) ** 2
+    if prec == "c128":
+        assert np.array_equal(got, exp)
+        assert np.max(np.abs(states - fin)) < 1e-12
+    else:
+        assert np.mean(got == exp) > 0.9
+        assert np.min(ov) > 1 - 1e-5, ov
+
+
 def test_depolarizing_statistics(eng):
     """X then depolarizing(p) on 1 qubit: P(1) = 1 - 2p/3."""
     p, T = 0.3, 20000
@@ -102,6 +131,28 @@ def test_execute_noisy_circuit_result_map(eng):
     assert np.allclose(r["density-matrix"], r["density-matrix"].conj().T)
     # an empty circuit fails (hardware_simulator_test.clj:133-141)
     assert execute_noisy_circuit(qc.create_circuit(1), kyoto_like, {"shots": 10})["job-status"] == "failed"
+
+
+def test_density_matrix_equals_trajectory_average(eng):
+    """rho = (1/k) sum_t psi_t psi_t^H (state.clj:726-800) over the returned
+    trajectories, on complex amplitudes (RY then S: rho differs from rho^T
+    and from conj(rho), so a dropped or extra transpose or conjugation fails)."""
+    circ = qc.create_circuit(2)
+    circ = qc.ry_gate(circ, 0, 0.7)
+    circ = qc.s_gate(circ, 0)
+    circ = qc.cnot_gate(circ, 0, 1)
+    circ = qc.ry_gate(circ, 1, 1.9)
+    circ = qc.t_gate(circ, 1)
+    model = {"gate-noise": {"cnot": {"noise-type": "depolarizing", "noise-strength": 0.2},
+                            "ry": {"noise-type": "amplitude-damping", "noise-strength": 0.3}}}
+    res = execute_noisy_circuit(circ, model, {"shots": 64, "seed": 8, "result-specs": {"density-matrix": True}})
+    r = res["results"]
+    states = np.array([np.asarray(t["state-vector"]) for t in r["trajectories"]])
+    k = len(states)
+    exp = np.einsum("ti,tj->ij", states, states.conj()) / k
+    rho = np.asarray(r["density-matrix"])
+    assert np.max(np.abs(rho - exp)) < 1e-13
+    assert np.max(np.abs(exp - exp.T)) > 1e-3          # the case tells rho from its transpose
 
 
 def test_trajectory_errors(eng):

@@ -73,7 +73,8 @@ def _random_program(rng, n, count, kinds1=None):
             a, b, c = rng.choice(n, 3, replace=False)
             rows.append((OP[rng.choice(["TOFFOLI", "FREDKIN"])], int(a), int(b), int(c), 0, 0.0))
         else:
-            rows.append((OP[rng.choice(["RYDBERG_BLOCKADE", "GLOBAL_RX", "GLOBAL_H", "GLOBAL_X"])],
+            rows.append((OP[rng.choice(["RYDBERG_BLOCKADE", "GLOBAL_RX", "GLOBAL_RY", "GLOBAL_RZ", "GLOBAL_H",
+                                        "GLOBAL_X", "GLOBAL_Y", "GLOBAL_Z"])],
                          0, 0, 0, int(rng.integers(1, 1 << min(n, 62))), ang))
     return qc.ops_from_tuples(rows)
 
@@ -88,6 +89,56 @@ def test_random_programs_vs_oracle(eng, oracle, n):
     assert np.max(np.abs(got - exp)) < TOL_C128
     got_msb = run_dev(eng, n, ops, flags=QSV_FLAG_MSB_SWAP)
     assert np.max(np.abs(got_msb - oracle.run(n, ops, flags=QSV_FLAG_MSB_SWAP))) < TOL_C128
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_global_gates_vs_oracle(eng, oracle, prec):
+    """Every global-* kind (gate.clj:1098-1253) on a complex random state, per
+    gate and through fused passes, against the oracle."""
+    from qclojure_amd._abi import QSV_FLAG_FUSE
+    rng = np.random.default_rng(41)
+    kinds = ["GLOBAL_RX", "GLOBAL_RY", "GLOBAL_RZ", "GLOBAL_H", "GLOBAL_X", "GLOBAL_Y", "GLOBAL_Z"]
+    for n in (1, 4, 13, 16):
+        init = oracle.run(n, _random_program(rng, n, 30))
+        for kind in kinds:
+            ops = qc.ops_from_tuples([(OP[kind], 0, 0, 0, 0, float(rng.uniform(0, 2 * math.pi)))])
+            exp = oracle.apply_program(init, n, ops)
+            for flags in (0, QSV_FLAG_FUSE):
+                note(f"global {kind} n={n} {prec} flags={flags}")
+                got = run_dev(eng, n, ops, init, prec=prec, flags=flags)
+                if prec == "c128":
+                    assert np.max(np.abs(got - exp)) < TOL_C128, (kind, n, flags)
+                else:
+                    assert abs(np.vdot(exp, got)) ** 2 > FID_C64 and np.max(np.abs(got - exp)) < 1e-6, \
+                        (kind, n, flags)
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_normalize_vs_oracle(eng, oracle, kats, prec):
+    """qsv_normalize (normalize-state, state.clj:524-551) on the reference's
+    cases, on unnormalised random states, and below the 1e-12 tolerance,
+    where the amplitudes are kept."""
+    for case in kats["normalize-cases"]:
+        with eng.DeviceState(case["n"], prec) as s:
+            s.upload(as_vec(case["init"]))
+            got = s.normalize().download()
+        assert np.max(np.abs(got - as_vec(case["expect"]))) < (case["tol"] if prec == "c128" else 1e-7)
+    rng = np.random.default_rng(43)
+    for n in (1, 5, 12, 20):
+        v = (rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)) * rng.uniform(0.01, 30.0)
+        with eng.DeviceState(n, prec) as s:
+            s.upload(v)
+            ref_in = s.download()          # the amplitudes as stored (rounded for c64)
+            got = s.normalize().download()
+            assert abs(s.norm2() - 1.0) < (1e-14 if prec == "c128" else 1e-6)
+        exp = oracle.normalize(ref_in, n)
+        tol = 1e-15 if prec == "c128" else 3e-7 * np.max(np.abs(exp))
+        assert np.max(np.abs(got - exp)) < tol, n
+    tiny = np.array([1e-13, 0.0, 0.0, 0.0], complex)
+    with eng.DeviceState(2, prec) as s:
+        s.upload(tiny)
+        got = s.normalize().download()
+    assert np.max(np.abs(got - tiny)) < (1e-28 if prec == "c128" else 1e-20)
 
 
 def test_literal_oracle_small(eng, oracle):

@@ -86,3 +86,31 @@ def test_bad_requests_fail_fast(eng):
             srv.submit(4, qc.ops_from_tuples([(OP["MEASURE"], 0, 0, 0, 1, 0.0)]), 10, 0)
         with pytest.raises(QsvError):
             srv.status(12345)
+
+
+def test_cancel(eng):
+    """cancel-job (ideal_simulator.clj:154-163) on the serving loop: a queued
+    request leaves the queue, an admitted one finishes as cancelled (or, if it
+    already finished, cannot be cancelled), unknown ids are not found."""
+    n = 22
+    budget = (1 << n) * 8                       # one 22-qubit complex64 state at a time
+    ops = wl.random_layer_ops(n, 100, seed=5)
+    with eng.Server("c64", hbm_budget_bytes=budget) as srv:
+        ids = [srv.submit(n, ops, 100, k) for k in range(4)]
+        assert srv.cancel(ids[3]) == "cancelled"      # behind three others of the same size: still queued
+        first = srv.cancel(ids[0])
+        assert first in ("cancelled", "cannot-cancel")
+        infos = [srv.wait(i, 100)[0] for i in ids[:3]]
+        info3, outs3 = srv.wait(ids[3], 100)
+        assert info3["status"] == "cancelled"
+        assert infos[0]["status"] == ("cancelled" if first == "cancelled" else "completed")
+        assert infos[1]["status"] == infos[2]["status"] == "completed"
+        assert srv.cancel(ids[1]) == "not-found"      # collected by wait
+        done = srv.submit(10, wl.random_layer_ops(10, 2, seed=1), 10, 1)
+        while srv.status(done)["status"] != "completed":
+            time.sleep(0.001)
+        assert srv.cancel(done) == "cannot-cancel"
+        assert srv.cancel(987654321) == "not-found"
+        st = srv.stats()
+        assert st["cancelled"] == 1 + (first == "cancelled")
+        assert st["hbm_in_use_bytes"] == 0

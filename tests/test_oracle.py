@@ -60,6 +60,49 @@ def test_shot_windows(kats, oracle):
         check_shots(case, outs)
 
 
+def test_probability_kats(kats, oracle):
+    """qs/probability on |0>, |1>, |+>, |00> (state_test.clj:145-166); exact
+    where the reference compares with =."""
+    for case in kats["probability-cases"]:
+        p = oracle.probabilities(as_vec(case["init"]), case["n"])
+        exp = np.asarray(case["probs"])
+        if case["tol"] == "exact":
+            assert np.array_equal(p, exp), (case["id"], p)
+        else:
+            assert np.max(np.abs(p - exp)) < case["tol"], (case["id"], p)
+
+
+def test_normalize_kats(kats, oracle):
+    """normalize-state on {3, 4} and |+> (state_test.clj:114-140)."""
+    for case in kats["normalize-cases"]:
+        out = oracle.normalize(as_vec(case["init"]), case["n"])
+        assert abs(np.sum(np.abs(out) ** 2) - 1.0) < case["norm2-tol"], case["id"]
+        assert np.max(np.abs(out - as_vec(case["expect"]))) < case["tol"], (case["id"], out)
+
+
+def test_probabilities_norm_and_normalize_closed_form(oracle):
+    """Born rule |a_i|^2 at index i (numpy's abs, so a dropped imaginary part,
+    a missing square or a bit-reversed index fails), sum = squared 2-norm, and
+    normalize = psi / ||psi|| except at or below the 1e-12 tolerance."""
+    rng = np.random.default_rng(17)
+    for n in range(1, 9):
+        v = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+        v *= rng.uniform(0.1, 10.0)
+        p = oracle.probabilities(v, n)
+        assert np.allclose(p, np.abs(v) ** 2, rtol=1e-14, atol=0)
+        assert abs(oracle.norm2(v, n) - np.linalg.norm(v) ** 2) < 1e-12 * np.linalg.norm(v) ** 2
+        out = oracle.normalize(v, n)
+        assert np.allclose(out, v / np.linalg.norm(v), rtol=1e-14, atol=1e-16)
+    # asymmetric single amplitude: only index 5 carries probability
+    v = np.zeros(8, complex)
+    v[5] = 0.6 + 0.8j
+    assert np.array_equal(oracle.probabilities(v, 3) > 0, np.arange(8) == 5)
+    # at or below the tolerance (and the zero vector) the amplitudes are kept
+    tiny = np.array([1e-13, 0.0], complex)
+    assert np.array_equal(oracle.normalize(tiny, 1), tiny)
+    assert np.array_equal(oracle.normalize(np.zeros(4, complex), 2), np.zeros(4, complex))
+
+
 def test_bits_to_index(kats):
     from qclojure_amd.result import _bits_to_index
     for c in kats["bits-to-index"]:
@@ -140,6 +183,32 @@ def test_global_x_phase(oracle):
         psi = oracle.run(n, qc.decode_circuit(qc.global_x_gate(qc.create_circuit(n))))
         assert abs(psi[-1] - (-1j) ** n) < 1e-14
         assert np.sum(np.abs(psi[:-1])) < 1e-14
+
+
+@pytest.mark.parametrize("literal", [True, False])
+def test_global_gates_match_kron(oracle, literal):
+    """global-rx/ry/rz/hadamard apply the gate to qubits 0..n-1 in turn, and
+    global-x/y/z are RX/RY/RZ(pi) on every qubit (gate.clj:1098-1253): the
+    n-fold numpy Kronecker power of the textbook matrix, on a complex state.
+    (A global-y taken as Pauli Y would differ by the phase (-i)^n.)"""
+    rng = np.random.default_rng(23)
+    H = np.array([[1, 1], [1, -1]], complex) / math.sqrt(2)
+
+    def rot(P, th):
+        return math.cos(th / 2) * I2 - 1j * math.sin(th / 2) * P
+
+    for n in (1, 2, 4, 6):
+        psi = rand_state(n, rng)
+        th = float(rng.uniform(0, 2 * math.pi))
+        cases = {"GLOBAL_RX": rot(X, th), "GLOBAL_RY": rot(Y, th), "GLOBAL_RZ": rot(Z, th), "GLOBAL_H": H,
+                 "GLOBAL_X": rot(X, math.pi), "GLOBAL_Y": rot(Y, math.pi), "GLOBAL_Z": rot(Z, math.pi)}
+        for kind, g in cases.items():
+            m = np.array([[1.0 + 0j]])
+            for _ in range(n):
+                m = np.kron(m, g)
+            ops = qc.ops_from_tuples([(OP[kind], 0, 0, 0, 0, th)])
+            got = oracle.apply_program(psi, n, ops, literal=literal)
+            assert np.max(np.abs(got - m @ psi)) < 1e-13, (n, kind)
 
 
 def test_pauli_expectation_matches_dense(oracle):
@@ -244,6 +313,15 @@ def test_measure_state_inverse_cdf(oracle):
         u = float(rng.uniform())
         exp = min(int(np.sum(cum < u * cum[-1])), (1 << n) - 1)
         assert oracle.lib().ref_measure_state(oracle._p(psi), n, u) == exp
+
+
+def test_sample_prefix_equals_sample(oracle):
+    """The binary-search sampler for large states gives ref_sample's shots."""
+    rng = np.random.default_rng(19)
+    for n in (1, 3, 8, 12):
+        psi = rand_state(n, rng)
+        psi[rng.integers(0, 1 << n, size=(1 << n) // 3)] = 0     # zero runs: equal running sums
+        assert np.array_equal(oracle.sample_prefix(psi, n, 3000, 5 + n), oracle.sample(psi, n, 3000, 5 + n))
 
 
 def test_qaoa_p1_closed_form(oracle):
