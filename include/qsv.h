@@ -132,6 +132,18 @@ int qsv_probabilities(const qsv_state* s, uint64_t offset, uint64_t count, doubl
 /* measure-state repeated `shots` times (result.clj:224, state.clj:869-913):
  * outcome k uses uniform u_k = qsv_uniform(seed, k). */
 int qsv_sample(const qsv_state* s, uint64_t shots, uint64_t seed, uint64_t* outcomes);
+/* qsv_sample without any device allocation. No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, hosts that forbid hipMalloc). The sampler's
+ * chunk prefix and the device copy of the outcomes live in a fixed per-GPU
+ * scratch array of 12 MiB, so the state may hold at most
+ * QSV_NOALLOC_MAX_QUBITS qubits and at most QSV_NOALLOC_MAX_SHOTS shots are
+ * drawn; larger requests return QSV_E_INVALID. Outcomes equal qsv_sample's
+ * for the same state, shots and seed. Calls that overlap on one GPU share
+ * the scratch and race: the caller serialises them. */
+#define QSV_NOALLOC_MAX_QUBITS 30
+#define QSV_NOALLOC_MAX_SHOTS  (1ull << 19)
+int qsv_sample_noalloc(const qsv_state* s, uint64_t shots, uint64_t seed, uint64_t* outcomes);
 /* measure-specific-qubits (state.clj:915-1014): collapse `mask` using uniform u.
  * outcome bit j = value of the j-th measured qubit in ascending qubit order;
  * probs (optional, 2^popcount(mask) doubles) receives every outcome probability. */

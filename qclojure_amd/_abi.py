@@ -79,6 +79,8 @@ class QsvTuneResult(ctypes.Structure):
 
 QSV_MAX_KRAUS = 4
 QSV_TRAJ_SLOTS = 1 << 20
+QSV_NOALLOC_MAX_QUBITS = 30
+QSV_NOALLOC_MAX_SHOTS = 1 << 19
 
 
 class QsvKrausSite(ctypes.Structure):
@@ -140,6 +142,7 @@ SIGNATURES = {
     "qsv_normalize": (_i32, [_vp]),
     "qsv_probabilities": (_i32, [_vp, _u64, _u64, _vp]),
     "qsv_sample": (_i32, [_vp, _u64, _u64, _vp]),
+    "qsv_sample_noalloc": (_i32, [_vp, _u64, _u64, _vp]),
     "qsv_measure_qubits": (_i32, [_vp, _u64, _dbl, _pu64, _vp]),
     "qsv_expect_pauli": (_i32, [_vp, ctypes.c_char_p, _u64, _vp]),
     "qsv_expect_1q": (_i32, [_vp, _i32, _vp, _vp]),

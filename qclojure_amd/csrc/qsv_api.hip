@@ -89,7 +89,33 @@ struct qsv_state {
   uint64_t bytes_per_amp() const { return prec == QSV_C128 ? 16 : 8; }
 };
 
+// Scratch of qsv_sample_noalloc: the FP64 chunk prefix of the sampler (one
+// partial sum per 2^kSampleChunkLog amplitudes, for states of up to
+// QSV_NOALLOC_MAX_QUBITS qubits) followed by the device copy of the outcomes
+// (up to QSV_NOALLOC_MAX_SHOTS). Every GPU the library runs on holds its own
+// copy until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device (g_scratch_ptr).
+constexpr uint64_t kNoAllocPrefixBytes = (1ull << (QSV_NOALLOC_MAX_QUBITS - kSampleChunkLog)) * sizeof(double);
+constexpr uint64_t kNoAllocShotBytes = QSV_NOALLOC_MAX_SHOTS * sizeof(uint64_t);
+static __device__ __align__(256) unsigned char scratch[kNoAllocPrefixBytes + kNoAllocShotBytes];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Resolves this device's copy of `scratch`; the device must be current.
+// Caller holds g_mu.
+int resolve_scratch_locked(int device)
+{
+  if (device < 0 || device >= kMaxDevices) return fail(QSV_E_INVALID, "device index beyond the scratch table");
+  if (g_scratch_ptr[device]) return QSV_OK;
+  void* p = nullptr;
+  HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) return fail(QSV_E_DEVICE, "hipGetSymbolAddress returned a null scratch address");
+  g_scratch_ptr[device] = p;
+  return QSV_OK;
+}
 
 int ensure_init()
 {
@@ -99,6 +125,8 @@ int ensure_init()
   HIPCHK(hipGetDeviceCount(&count));
   if (count <= 0) return fail(QSV_E_DEVICE, "no HIP device visible");
   HIPCHK(hipSetDevice(g_device));
+  int rc = resolve_scratch_locked(g_device);
+  if (rc) return rc;
   g_init = true;
   return QSV_OK;
 }
@@ -873,6 +901,8 @@ int qsv_init(int device)
   HIPCHK(hipGetDeviceCount(&count));
   if (device < 0 || device >= count) return fail(QSV_E_INVALID, "device index out of range");
   HIPCHK(hipSetDevice(device));
+  int rc = resolve_scratch_locked(device);
+  if (rc) return rc;
   g_device = device;
   g_init = true;
   return QSV_OK;
@@ -1180,6 +1210,37 @@ int qsv_sample(const qsv_state* cs, uint64_t shots, uint64_t seed, uint64_t* out
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   if (dout) hipFree(dout);
   if (e != hipSuccess) return fail(QSV_E_DEVICE, std::string("sample: ") + hipGetErrorString(e));
+  return QSV_OK;
+}
+
+int qsv_sample_noalloc(const qsv_state* s, uint64_t shots, uint64_t seed, uint64_t* outcomes)
+{
+  int rc = check_state(s);
+  if (rc) return rc;
+  if (shots && !outcomes) return fail(QSV_E_INVALID, "null outcomes");
+  if (s->n > QSV_NOALLOC_MAX_QUBITS)
+    return fail(QSV_E_INVALID, "qsv_sample_noalloc: " + std::to_string(s->n) + " qubits exceed the scratch limit of " +
+                                   std::to_string(QSV_NOALLOC_MAX_QUBITS));
+  if (shots > QSV_NOALLOC_MAX_SHOTS)
+    return fail(QSV_E_INVALID, "qsv_sample_noalloc: " + std::to_string(shots) + " shots exceed the scratch limit of " +
+                                   std::to_string(QSV_NOALLOC_MAX_SHOTS));
+  void* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    rc = resolve_scratch_locked(s->device);  // s->device is current (check_state)
+    base = g_scratch_ptr[s->device];
+  }
+  if (rc) return rc;
+  // the sampler overwrites every chunk sum and every outcome it reads back,
+  // so nothing in the scratch needs clearing between calls
+  double* prefix = static_cast<double*>(base);
+  uint64_t* dout = reinterpret_cast<uint64_t*>(static_cast<unsigned char*>(base) + kNoAllocPrefixBytes);
+  launch_sample(s->prec, s->amps, s->S, prefix, shots, seed, dout, s->stream);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && shots)
+    e = hipMemcpyAsync(outcomes, dout, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  if (e != hipSuccess) return fail(QSV_E_DEVICE, std::string("sample_noalloc: ") + hipGetErrorString(e));
   return QSV_OK;
 }
 
