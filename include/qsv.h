@@ -247,6 +247,9 @@ typedef struct qsv_request_info {
   double   queue_ms;          /* submit -> admitted                                 */
   double   latency_ms;        /* submit -> results on the host                      */
   char     error[128];
+  double   norm2;             /* completed: FP64 sum |a|^2 of the final state (the
+                                 sampler's total); |norm2 - 1| is the request's
+                                 norm drift. NaN otherwise.                          */
 } qsv_request_info;
 
 typedef struct qsv_server_stats {

@@ -106,7 +106,8 @@ class QsvServerConfig(ctypes.Structure):
 
 class QsvRequestInfo(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int32), ("num_qubits", ctypes.c_int32), ("shots", ctypes.c_uint64),
-                ("queue_ms", ctypes.c_double), ("latency_ms", ctypes.c_double), ("error", ctypes.c_char * 128)]
+                ("queue_ms", ctypes.c_double), ("latency_ms", ctypes.c_double), ("error", ctypes.c_char * 128),
+                ("norm2", ctypes.c_double)]
 
 
 class QsvServerStats(ctypes.Structure):

@@ -43,6 +43,24 @@ class QuantumBackend(abc.ABC):
     def submit_circuit(self, circuit: dict, options: dict | None = None) -> str: ...
 
     @abc.abstractmethod
+    def _precision_report(self, dev: DeviceState, norm_in, ops, flags: int, seed: int, init) -> dict:
+        """What a complex64 result is worth: the drift of its FP64 norm over
+        the program, and (up to shadow-max-qubits) the overlap fidelity
+        |<psi32|psi64>|^2 (state-fidelity of state.clj:1164-1185, squared) and
+        max |a32 - a64| against the same program run on a complex128 state."""
+        if dev.precision != _abi.QSV_C64:
+            return {"precision": "c128"}
+        rep = {"precision": "c64", "norm-drift": abs(dev.norm2() - norm_in)}
+        if dev.num_qubits <= int(self.config.get("shadow-max-qubits", 24)):
+            with DeviceState(dev.num_qubits, "c128") as sh:
+                if init is not None:
+                    sh.upload(np.asarray(init["state-vector"], dtype=np.complex128))
+                sh.apply(ops, flags, seed)
+                c = dev.compare(sh)
+            rep.update({"fidelity-vs-c128": c["fidelity"], "max-abs-amplitude-error": c["max_abs_diff"],
+                        "reference": "complex128-device-shadow"})
+        return rep
+
     def job_status(self, job_id: str) -> str: ...
 
     @abc.abstractmethod
@@ -120,7 +138,11 @@ class GpuSimulator(QuantumBackend, BatchJobBackend):
     reference's LSB positions), "keep-final-state" (default True: the result's
     :final-state and lazily read vectors stay in HBM until read or until
     cleanup_completed_jobs drops the job; False: they are copied to host memory
-    when the job completes and the device state is freed at once).
+    when the job completes and the device state is freed at once),
+    "fuse" (default True: fused LDS-tiled passes), "shadow-max-qubits"
+    (default 24: complex64 jobs up to this size also run the program on a
+    complex128 state and report |<psi32|psi64>|^2 and max |d amplitude|
+    against it in the result's "precision-report"; 0 turns the shadow off).
     """
 
     def __init__(self, config: dict | None = None):
@@ -190,12 +212,16 @@ class GpuSimulator(QuantumBackend, BatchJobBackend):
                     raise CircuitError("initial state and circuit have different numbers of qubits")
                 dev.upload(np.asarray(init["state-vector"], dtype=np.complex128))
             flags = _abi.QSV_FLAG_MSB_SWAP if self.config.get("msb-swap") else 0
+            if self.config.get("fuse", True):
+                flags |= _abi.QSV_FLAG_FUSE
+            norm_in = dev.norm2() if dev.precision == _abi.QSV_C64 else None
             dev.apply(ops, flags, seed)
             specs = options.get("result-specs") or {}
             base = {"final-state": FinalState(dev),
                     "result-types": set(specs.keys()),
                     "circuit": circuit,
-                    "circuit-metadata": circuit_metadata(circuit)}
+                    "circuit-metadata": circuit_metadata(circuit),
+                    "precision-report": self._precision_report(dev, norm_in, ops, flags, seed, init)}
             results = extract_results(base, specs, dev, seed ^ 0xA5A5A5A5A5A5A5A5) if specs else base
             dev.sync()
             if not self.config.get("keep-final-state", True):
@@ -206,6 +232,24 @@ class GpuSimulator(QuantumBackend, BatchJobBackend):
             return {"job-status": "failed", "error-message": str(e),
                     "exception-type": type(e).__name__,
                     "trace": traceback.format_exc(limit=3) if os.environ.get("QSV_DEBUG") else None}
+
+    def _precision_report(self, dev: DeviceState, norm_in, ops, flags: int, seed: int, init) -> dict:
+        """What a complex64 result is worth: the drift of its FP64 norm over
+        the program, and (up to shadow-max-qubits) the overlap fidelity
+        |<psi32|psi64>|^2 (state-fidelity of state.clj:1164-1185, squared) and
+        max |a32 - a64| against the same program run on a complex128 state."""
+        if dev.precision != _abi.QSV_C64:
+            return {"precision": "c128"}
+        rep = {"precision": "c64", "norm-drift": abs(dev.norm2() - norm_in)}
+        if dev.num_qubits <= int(self.config.get("shadow-max-qubits", 24)):
+            with DeviceState(dev.num_qubits, "c128") as sh:
+                if init is not None:
+                    sh.upload(np.asarray(init["state-vector"], dtype=np.complex128))
+                sh.apply(ops, flags, seed)
+                c = dev.compare(sh)
+            rep.update({"fidelity-vs-c128": c["fidelity"], "max-abs-amplitude-error": c["max_abs_diff"],
+                        "reference": "complex128-device-shadow"})
+        return rep
 
     def job_status(self, job_id: str) -> str:
         with _STATE_LOCK:

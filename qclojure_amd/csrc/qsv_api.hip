@@ -86,6 +86,7 @@ struct qsv_state {
   uint64_t* samp_dev = nullptr;   // serving: reusable shot buffers (grow only)
   uint64_t* samp_host = nullptr;  // pinned
   uint64_t samp_cap = 0;
+  double* norm_host = nullptr;    // serving: pinned FP64 sum |a|^2 read back after sampling
   uint64_t bytes_per_amp() const { return prec == QSV_C128 ? 16 : 8; }
 };
 
@@ -1977,7 +1978,7 @@ int qsv_noisy_trajectories(int num_qubits, int precision, const qsv_op* ops, uin
         return rc;
       }
       launch_lds_batch(precision, (unsigned)c, S * B, d_insts, d_recs, nullptr, nullptr, d_order, nullptr, arena, st);
-      launch_batch_sample(precision, arena, d_jobs, (unsigned)c, d_out, st);
+      launch_batch_sample(precision, arena, d_jobs, (unsigned)c, d_out, nullptr, st);
       if (hipMemcpyAsync(outcomes + t0, d_out, c * sizeof(uint64_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
           hipStreamSynchronize(st) != hipSuccess) {
         cleanup();
@@ -2040,6 +2041,7 @@ struct SrvReq {
   int status = QSV_REQ_QUEUED;
   std::string err;
   std::vector<uint64_t> outcomes;
+  double norm2 = NAN;       // FP64 sum |a|^2 of the final state, from the sampler
   uint64_t bytes = 0;
   bool small = false;
   bool cancel = false;      // cancelled while admitted: finishes as QSV_REQ_CANCELLED
@@ -2059,6 +2061,8 @@ struct SmallSlot {
   SampleJob* jobs = nullptr;   size_t jobs_cap = 0;
   uint64_t* d_out = nullptr;   size_t out_cap = 0;
   uint64_t* h_out = nullptr;   size_t hout_cap = 0;
+  double* d_norm = nullptr;    size_t norm_cap = 0;      // FP64 sum |a|^2 per request
+  double* h_norm = nullptr;    size_t hnorm_cap = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev = nullptr;
   bool busy = false;
@@ -2229,6 +2233,14 @@ int launch_small(qsv_server* sv, std::vector<SrvReq*>& batch)
   if ((rc = grow_dev(&slot->order, &slot->order_cap, order.size()))) return rc;
   if ((rc = grow_dev(&slot->jobs, &slot->jobs_cap, jobs.size()))) return rc;
   if ((rc = grow_dev(&slot->d_out, &slot->out_cap, std::max<uint64_t>(nout, 1)))) return rc;
+  if ((rc = grow_dev(&slot->d_norm, &slot->norm_cap, jobs.size()))) return rc;
+  if (slot->hnorm_cap < jobs.size()) {
+    if (slot->h_norm) HIPCHK(hipHostFree(slot->h_norm));
+    slot->h_norm = nullptr;
+    const size_t c = std::max<size_t>(jobs.size(), slot->hnorm_cap * 2);
+    HIPCHK(hipHostMalloc((void**)&slot->h_norm, c * sizeof(double), hipHostMallocDefault));
+    slot->hnorm_cap = c;
+  }
   if (slot->hout_cap < nout) {
     if (slot->h_out) HIPCHK(hipHostFree(slot->h_out));
     slot->h_out = nullptr;
@@ -2247,9 +2259,10 @@ int launch_small(qsv_server* sv, std::vector<SrvReq*>& batch)
   if (setup_lds_batch(prec, lds) != 0) return fail(QSV_E_DEVICE, "serving: LDS reservation failed");
   launch_lds_batch(prec, (unsigned)batch.size(), lds, slot->insts, slot->recs, nullptr, nullptr, slot->order, nullptr,
                    slot->arena, slot->stream);
-  launch_batch_sample(prec, slot->arena, slot->jobs, (unsigned)jobs.size(), slot->d_out, slot->stream);
+  launch_batch_sample(prec, slot->arena, slot->jobs, (unsigned)jobs.size(), slot->d_out, slot->d_norm, slot->stream);
   if (nout)
     HIPCHK(hipMemcpyAsync(slot->h_out, slot->d_out, nout * sizeof(uint64_t), hipMemcpyDeviceToHost, slot->stream));
+  HIPCHK(hipMemcpyAsync(slot->h_norm, slot->d_norm, jobs.size() * sizeof(double), hipMemcpyDeviceToHost, slot->stream));
   HIPCHK(hipEventRecord(slot->ev, slot->stream));
   HIPCHK(hipGetLastError());
   slot->reqs = batch;
@@ -2284,11 +2297,15 @@ int launch_large(qsv_server* sv, SrvReq* r)
     HIPCHK(hipHostMalloc((void**)&s->samp_host, cap * sizeof(uint64_t), hipHostMallocDefault));
     s->samp_cap = cap;
   }
+  if (!s->norm_host) HIPCHK(hipHostMalloc((void**)&s->norm_host, sizeof(double), hipHostMallocDefault));
   r->d_out = s->samp_dev;
   r->h_out = s->samp_host;
   launch_sample(s->prec, s->amps, s->S, s->prefix, r->shots, r->seed, r->d_out, s->stream);
   if (r->shots)
     HIPCHK(hipMemcpyAsync(r->h_out, r->d_out, r->shots * sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream));
+  // the last inclusive chunk sum is the state's FP64 sum |a|^2
+  const uint64_t nch = std::max<uint64_t>(1, s->S >> std::min<int>(s->n, kSampleChunkLog));
+  HIPCHK(hipMemcpyAsync(s->norm_host, s->prefix + (nch - 1), sizeof(double), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipEventCreateWithFlags(&r->ev, hipEventDisableTiming));
   HIPCHK(hipEventRecord(r->ev, s->stream));
   HIPCHK(hipGetLastError());
@@ -2367,7 +2384,10 @@ void server_loop(qsv_server* sv)
       SrvReq* r = sv->running_large[k];
       const int pr = poll_event(r->ev, &perr);
       if (pr != POLL_PENDING) {
-        if (pr == POLL_DONE) r->outcomes.assign(r->h_out, r->h_out + r->shots);
+        if (pr == POLL_DONE) {
+          r->outcomes.assign(r->h_out, r->h_out + r->shots);
+          r->norm2 = *r->st->norm_host;
+        }
         release_large(sv, r, pr == POLL_FAILED);
         finish(sv, r, pr == POLL_DONE ? QSV_REQ_COMPLETED : QSV_REQ_FAILED, pr == POLL_DONE ? "" : perr);
         sv->running_large[k] = sv->running_large.back();
@@ -2385,6 +2405,7 @@ void server_loop(qsv_server* sv)
         SrvReq* r = slot.reqs[k];
         if (pr == POLL_DONE) {
           r->outcomes.assign(slot.h_out + slot.out_offsets[k], slot.h_out + slot.out_offsets[k] + r->shots);
+          r->norm2 = slot.h_norm[k];
           finish(sv, r, QSV_REQ_COMPLETED, "");
         } else {
           finish(sv, r, QSV_REQ_FAILED, perr);
@@ -2421,6 +2442,7 @@ void fill_info(const SrvReq* r, qsv_request_info* info)
   if (r->status != QSV_REQ_QUEUED) info->queue_ms = ms(r->t_submit, r->t_admit);
   if (r->status >= QSV_REQ_COMPLETED) info->latency_ms = ms(r->t_submit, r->t_done);
   std::snprintf(info->error, sizeof(info->error), "%s", r->err.c_str());
+  info->norm2 = r->status == QSV_REQ_COMPLETED ? r->norm2 : NAN;
 }
 
 }  // namespace

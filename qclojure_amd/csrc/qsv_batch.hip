@@ -149,7 +149,8 @@ __device__ __forceinline__ double uni01(uint64_t seed, uint64_t k)
 template <typename R>
 __global__ __launch_bounds__(256) void k_batch_sample(const typename BV<R>::T* __restrict__ arena,
                                                       const SampleJob* __restrict__ jobs,
-                                                      uint64_t* __restrict__ outcomes)
+                                                      uint64_t* __restrict__ outcomes,
+                                                      double* __restrict__ norms)
 {
   __shared__ double cum[256];
   const SampleJob jb = jobs[blockIdx.x];
@@ -173,6 +174,7 @@ __global__ __launch_bounds__(256) void k_batch_sample(const typename BV<R>::T* _
     __syncthreads();
   }
   const double total = cum[nch - 1];
+  if (norms && tid == 0) norms[blockIdx.x] = total;
   for (uint64_t s = tid; s < jb.shots; s += 256) {
     const double r = uni01(jb.seed, s) * total;
     uint32_t lo = 0, hi = nch;
@@ -195,15 +197,15 @@ __global__ __launch_bounds__(256) void k_batch_sample(const typename BV<R>::T* _
 }
 
 void launch_batch_sample(int prec, const void* arena, const void* jobs, unsigned njobs, uint64_t* outcomes,
-                         hipStream_t st)
+                         double* norms, hipStream_t st)
 {
   if (njobs == 0) return;
   if (prec == QSV_C128)
     hipLaunchKernelGGL((k_batch_sample<double>), dim3(njobs), dim3(256), 0, st, (const BV<double>::T*)arena,
-                       (const SampleJob*)jobs, outcomes);
+                       (const SampleJob*)jobs, outcomes, norms);
   else
     hipLaunchKernelGGL((k_batch_sample<float>), dim3(njobs), dim3(256), 0, st, (const BV<float>::T*)arena,
-                       (const SampleJob*)jobs, outcomes);
+                       (const SampleJob*)jobs, outcomes, norms);
 }
 
 // <H> of a Z-string Hamiltonian on a state in global memory: per-block FP64

@@ -199,7 +199,8 @@ void launch_zexpect_group(int prec, const void* base, uint64_t S, unsigned count
                           const double* tc, const uint64_t* trange, double* partials, unsigned blocks_per_state,
                           const uint64_t* dest, double* out, hipStream_t st);
 
-// Shots for many small states stored back to back in an arena.
+// Shots for many small states stored back to back in an arena; norms
+// (optional) receives each state's FP64 sum of |a|^2.
 struct SampleJob {
   uint64_t state_offset;   // amplitude offset in the arena
   uint64_t out_offset;     // outcome offset
@@ -209,7 +210,7 @@ struct SampleJob {
   int32_t pad;
 };
 void launch_batch_sample(int prec, const void* arena, const void* jobs, unsigned njobs, uint64_t* outcomes,
-                         hipStream_t st);
+                         double* norms, hipStream_t st);
 // Shared with the linear-algebra module (qsv_la.hip): set the calling
 // thread's error message and return `code`; make sure the runtime is
 // initialised; reach a state's amplitude buffer and stream.

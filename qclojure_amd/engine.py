@@ -290,8 +290,13 @@ class Server:
 
     @staticmethod
     def _info(info) -> dict:
-        return {"status": _abi.REQ_STATUS[info.status], "num-qubits": info.num_qubits, "shots": info.shots,
-                "queue-ms": info.queue_ms, "latency-ms": info.latency_ms, "error": info.error.decode()}
+        out = {"status": _abi.REQ_STATUS[info.status], "num-qubits": info.num_qubits, "shots": info.shots,
+               "queue-ms": info.queue_ms, "latency-ms": info.latency_ms, "error": info.error.decode()}
+        if info.status == 2:
+            # the served state's FP64 sum |a|^2 before sampling: its drift from 1
+            out["norm2"] = info.norm2
+            out["norm-drift"] = abs(info.norm2 - 1.0)
+        return out
 
     def status(self, rid: int) -> dict:
         info = _abi.QsvRequestInfo()

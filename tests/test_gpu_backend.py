@@ -228,3 +228,42 @@ def test_full_dimension_observables(oracle, prec):
         assert abs(v["variance-value"] - var) < 10 * tol
         with pytest.raises(ValueError):
             rs.extract_expectation_results(s, [np.eye(8)])
+
+
+def test_precision_report_on_c64_results(oracle):
+    """Every complex64 job states what it is worth: the FP64 norm drift of
+    its state over the program and, below the shadow size, the overlap
+    fidelity |<psi32|psi64>|^2 and max |d amplitude| against a complex128 run
+    of the same program; checked on the 12-qubit, 20-layer random circuit,
+    with the complex128 shadow itself against the oracle."""
+    be.reset_simulator_state()
+    circ = wl.random_layer_circuit(12, 20, seed=1)
+    sim64 = be.create_simulator({"seed": 7, "precision": "c64"})
+    sim128 = be.create_simulator({"seed": 7})
+    try:
+        r = be.execute_circuit(sim64, circ, {"result-specs": {"measurements": {"shots": 1024}}})
+        assert r["job-status"] == "completed", r
+        rep = r["results"]["precision-report"]
+        assert rep["precision"] == "c64" and rep["reference"] == "complex128-device-shadow"
+        assert rep["fidelity-vs-c128"] >= 1 - 1e-6
+        assert rep["max-abs-amplitude-error"] < 1e-6
+        assert rep["norm-drift"] < 1e-6
+        # the complex128 shadow is the oracle's state: the reported figures
+        # equal those of the c64 state against the oracle
+        psi = oracle.run(12, qc.decode_circuit(circ))
+        got = np.asarray(r["results"]["final-state"]["state-vector"])
+        assert abs(abs(np.vdot(psi, got)) ** 2 - rep["fidelity-vs-c128"]) < 1e-9
+        assert abs(np.max(np.abs(got - psi)) - rep["max-abs-amplitude-error"]) < 1e-9
+        r2 = be.execute_circuit(sim128, circ, {"result-specs": {}})
+        assert r2["results"]["precision-report"] == {"precision": "c128"}
+        # above the shadow size only the norm drift is reported
+        big = be.create_simulator({"precision": "c64", "shadow-max-qubits": 10})
+        try:
+            r3 = be.execute_circuit(big, circ, {"result-specs": {}})
+            assert set(r3["results"]["precision-report"]) == {"precision", "norm-drift"}
+        finally:
+            big.close()
+    finally:
+        sim64.close()
+        sim128.close()
+        be.reset_simulator_state()

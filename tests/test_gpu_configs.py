@@ -180,8 +180,10 @@ def test_c5_serving_large_requests(eng, oracle):
         st = srv.stats()
     assert st["completed"] == len(reqs) and st["failed"] == 0
     for (kind, n, ops, shots, seed), (info, outs) in zip(reqs, got):
-        note(f"C5: {kind} n={n} {info['status']} latency {info['latency-ms']:.1f} ms")
+        note(f"C5: {kind} n={n} {info['status']} latency {info['latency-ms']:.1f} ms "
+             f"norm drift {info.get('norm-drift')}")
         assert info["status"] == "completed" and len(outs) == shots
+        assert info["norm-drift"] < 1e-6
         if kind == "oracle":
             psi = oracle.run(n, ops)
             ref = oracle.sample_prefix(psi, n, shots, seed)

@@ -42,6 +42,8 @@ def test_same_alone_and_in_the_mix(eng, prec):
     assert st["small_batched_requests"] == sum(1 for r in reqs if r[0] <= lds_n)
     for (n, ops, shots, seed), (info, outs) in zip(reqs, got):
         assert info["status"] == "completed" and info["latency-ms"] > 0
+        # every served result states its FP64 norm drift
+        assert info["norm-drift"] < (1e-12 if prec == "c128" else 1e-6), (n, info)
         ref, psi = _alone(eng, n, ops, shots, seed, prec)
         # the batched sampler sums chunks in another order: allow rare boundary flips
         assert np.mean(outs == ref) > 0.99, (n, np.mean(outs == ref))
