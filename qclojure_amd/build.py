@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libqsv.so")
 ARCH = os.environ.get("QSV_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["qsv_api.hip", "qsv_gates.hip", "qsv_fuse.hip", "qsv_kernels.hip", "qsv_batch.hip", "qsv_la.hip"]
+SOURCES = ["qsv_api.hip", "qsv_gates.hip", "qsv_pass.hip", "qsv_kernels.hip", "qsv_batch.hip", "qsv_la.hip"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -71,8 +71,8 @@ struct qsv_state {
   qsv_kernel_stat stats[QSV_KCLASS_COUNT] = {};
   // fused-pass descriptor ring (pinned host staging + device copy per slot)
   struct PassSlot {
-    qsv::FPass* host = nullptr;
-    qsv::FPass* dev = nullptr;
+    qsv::PProg* host = nullptr;
+    qsv::PProg* dev = nullptr;
     size_t cap = 0;
     hipEvent_t event = nullptr;
   };
@@ -539,7 +539,107 @@ int run_direct(qsv_state* s, const BGate& g)
 // Pass descriptors travel through a small ring of pinned host + device
 // buffers; a slot is reused only after the event recorded behind its last
 // pass has completed.
-int upload_passes(qsv_state* s, const std::vector<FPass>& passes, FPass** dev)
+// ---- compiling a planned pass into the micro-op program of k_pass ---------
+// Local bit b's rank among the set bits of mask (b must be in mask).
+inline uint32_t rank_in(uint32_t mask, int b) { return __builtin_popcount(mask & ((1u << b) - 1u)); }
+
+// The two tile bits that thread-id bits 6 and 7 (the wave index) map to in
+// layout reg: the two highest bits outside the register mask.
+inline uint32_t wave_bits(uint32_t reg)
+{
+  uint32_t nonreg = ((1u << kPassTileBits) - 1u) & ~reg;
+  uint32_t out = 0;
+  for (int k = 0; k < 2; ++k) {
+    const uint32_t top = 1u << (31 - __builtin_clz(nonreg));
+    out |= top;
+    nonreg &= ~top;
+  }
+  return out;
+}
+
+// Resolve every gate of p against the register layout it runs in: register
+// bits become slot indices, thread bits thread-id bits (thread t holds local
+// bits pdep(t, ~reg)). Returns the byte length of the program.
+size_t compile_pass(const FPass& p, PProg& out)
+{
+  constexpr uint32_t FULL = (1u << kPassTileBits) - 1u;
+  std::memset(&out.h, 0, sizeof(out.h));
+  out.h.renorm = p.renorm;
+  out.h.load_mask = p.load_mask;
+  out.h.store_mask = p.store_mask;
+  std::memcpy(out.h.pos, p.pos, sizeof(out.h.pos));
+  std::memcpy(out.h.sorted_pos, p.sorted_pos, sizeof(out.h.sorted_pos));
+  int n = 0;
+  uint32_t reg = p.load_mask;
+  auto emit = [&]() -> POp& {
+    POp& op = out.ops[n++];
+    std::memset(&op, 0, sizeof(op));
+    return op;
+  };
+  for (int s = 0; s < p.nstages; ++s) {
+    const FStage& st = p.stages[s];
+    if (st.regmask != reg) {
+      POp& op = emit();
+      op.code = PO_RELAYOUT;
+      op.aux = st.regmask;
+      op.only1 = wave_bits(reg) == wave_bits(st.regmask) ? 1 : 0;
+      reg = st.regmask;
+    }
+    const uint32_t nonreg = FULL & ~reg;
+    auto slot_mask = [&](uint32_t local) {
+      uint32_t m = 0;
+      for (int b = 0; b < kPassTileBits; ++b)
+        if (((local & reg) >> b) & 1u) m |= 1u << rank_in(reg, b);
+      return m;
+    };
+    auto tid_mask = [&](uint32_t local) {
+      uint32_t m = 0;
+      for (int b = 0; b < kPassTileBits; ++b)
+        if (((local & nonreg) >> b) & 1u) m |= 1u << rank_in(nonreg, b);
+      return m;
+    };
+    for (int gi = st.g0; gi < st.g1; ++gi) {
+      const FGate& g = p.gates[gi];
+      POp& op = emit();
+      std::memcpy(op.m, g.m, sizeof(op.m));
+      for (int k = 0; k < 8; ++k) op.f[k] = (float)g.m[k];
+      switch (g.kind) {
+      case FG_DENSE:
+        op.code = PO_DENSE + 4 * (int)rank_in(reg, g.a) + g.sub;
+        op.kc = slot_mask(g.cmask);
+        op.tmask = tid_mask(g.cmask);
+        break;
+      case FG_PHASE:
+        if ((reg >> g.a) & 1u) {
+          op.code = PO_PHASE + 4 * (g.sub ? 1 : 0) + (int)rank_in(reg, g.a);
+        } else {
+          op.code = PO_PHASE_T;
+          op.aux = 1u << rank_in(nonreg, g.a);
+          op.only1 = g.sub ? 1 : 0;
+        }
+        op.kc = slot_mask(g.cmask);
+        op.tmask = tid_mask(g.cmask);
+        break;
+      case FG_BLOCKADE:
+        op.code = PO_BLOCKADE;
+        op.kc = slot_mask(g.bmask);
+        op.aux = tid_mask(g.bmask);
+        break;
+      default:   // FG_SWAP: local bits and local controls
+        op.code = PO_SWAP;
+        op.kc = (uint32_t)g.a;
+        op.aux = (uint32_t)g.b;
+        op.tmask = g.cmask;
+        break;
+      }
+    }
+  }
+  out.h.nops = n;
+  out.h.store_local = (reg != p.store_mask && wave_bits(reg) == wave_bits(p.store_mask)) ? 1 : 0;
+  return offsetof(PProg, ops) + (size_t)n * sizeof(POp);
+}
+
+int upload_passes(qsv_state* s, const std::vector<FPass>& passes, PProg** dev)
 {
   auto& slot = s->ring[s->ring_next];
   s->ring_next = (s->ring_next + 1) % qsv_state::kRing;
@@ -551,13 +651,13 @@ int upload_passes(qsv_state* s, const std::vector<FPass>& passes, FPass** dev)
     slot.host = nullptr;
     slot.dev = nullptr;
     const size_t cap = std::max<size_t>(need, 64);
-    HIPCHK(hipHostMalloc((void**)&slot.host, cap * sizeof(FPass), hipHostMallocDefault));
-    HIPCHK(hipMalloc((void**)&slot.dev, cap * sizeof(FPass)));
+    HIPCHK(hipHostMalloc((void**)&slot.host, cap * sizeof(PProg), hipHostMallocDefault));
+    HIPCHK(hipMalloc((void**)&slot.dev, cap * sizeof(PProg)));
     slot.cap = cap;
   }
   if (!slot.event) HIPCHK(hipEventCreateWithFlags(&slot.event, hipEventDisableTiming));
-  std::memcpy(slot.host, passes.data(), need * sizeof(FPass));
-  HIPCHK(hipMemcpyAsync(slot.dev, slot.host, need * sizeof(FPass), hipMemcpyHostToDevice, s->stream));
+  for (size_t i = 0; i < need; ++i) compile_pass(passes[i], slot.host[i]);
+  HIPCHK(hipMemcpyAsync(slot.dev, slot.host, need * sizeof(PProg), hipMemcpyHostToDevice, s->stream));
   *dev = slot.dev;
   s->ring_pending = &slot;
   return QSV_OK;
@@ -573,13 +673,13 @@ int launch_passes(qsv_state* s, std::vector<FPass>& passes, bool f32)
     if (!s->norm_buf) HIPCHK(hipMalloc((void**)&s->norm_buf, (2 * ntiles + 2) * sizeof(double)));
     nb = TileNorm{s->norm_buf, s->norm_buf + ntiles, s->norm_buf + 2 * ntiles};
   }
-  FPass* dev = nullptr;
+  PProg* dev = nullptr;
   int rc = upload_passes(s, passes, &dev);
   if (rc) return rc;
   for (size_t i = 0; i < passes.size(); ++i) {
-    const FPass* p = dev + i;
+    const PProg* p = dev + i;
     rc = timed(s, QSV_KCLASS_BLOCK, 2.0 * s->S * s->bytes_per_amp(),
-               [&] { launch_tile(s->prec, f32, s->amps, p, ntiles, nb, s->stream); });
+               [&] { launch_pass(s->prec, f32, s->amps, p, ntiles, nb, s->stream); });
     if (rc) return rc;
     if (passes[i].renorm & FP_RENORM_OUT) {
       const bool first = (passes[i].renorm & FP_RENORM_IN) != 0;
@@ -808,11 +908,151 @@ int fuse_walk(const qsv_state* s, const std::vector<BGate>& gs, bool f32, Emit&&
   return QSV_OK;
 }
 
+// ---- gate folding on the host (fused and batched paths) -------------------
+// 2x2 complex products, row-major (re, im) pairs: c = a b.
+void mul2(const double a[8], const double b[8], double c[8])
+{
+  for (int r = 0; r < 2; ++r)
+    for (int col = 0; col < 2; ++col) {
+      double re = 0, im = 0;
+      for (int k = 0; k < 2; ++k) {
+        const double ar = a[2 * (2 * r + k)], ai = a[2 * (2 * r + k) + 1];
+        const double br = b[2 * (2 * k + col)], bi = b[2 * (2 * k + col) + 1];
+        re += ar * br - ai * bi;
+        im += ar * bi + ai * br;
+      }
+      c[2 * (2 * r + col)] = re;
+      c[2 * (2 * r + col) + 1] = im;
+    }
+}
+
+// a b == b a exactly (diagonal pairs; X with an X-axis rotation; ...).
+bool commutes_exactly(const double a[8], const double b[8])
+{
+  double ab[8], ba[8];
+  mul2(a, b, ab);
+  mul2(b, a, ba);
+  for (int k = 0; k < 8; ++k)
+    if (!(ab[k] == ba[k])) return false;
+  return true;
+}
+
+bool is_exchange(const double m[8])
+{
+  return m[0] == 0 && m[1] == 0 && m[2] == 1 && m[3] == 0 && m[4] == 1 && m[5] == 0 && m[6] == 0 && m[7] == 0;
+}
+
+void set_1q(BGate& g, const double m[8])
+{
+  std::memcpy(g.m, m, sizeof(g.m));
+  g.kind = is_diag(m) ? FG_PHASE : FG_DENSE;
+  g.phase_only = g.kind == FG_PHASE && is_one(m[0], m[1]);
+}
+
+bool is_1q(const BGate& g) { return (g.kind == FG_DENSE || g.kind == FG_PHASE) && g.cbits.empty(); }
+
+// Does the one-qubit matrix m on bit b commute with gate x (which touches b)?
+bool commutes_on_bit(const BGate& x, int b, const double m[8])
+{
+  if (x.kind == FG_DENSE || x.kind == FG_PHASE) {
+    for (int c : x.cbits)
+      if (c == b) return is_diag(m);   // b is a control: only diagonals commute with its projectors
+    return commutes_exactly(m, x.m);   // b is the target
+  }
+  if (x.kind == FG_BLOCKADE) return is_diag(m);
+  return false;
+}
+
+// Rewrite a decoded program into an equivalent one with fewer gates, exactly
+// in the mathematics (floating-point rounding aside):
+//  * CNOT(c,t) D CNOT(c,t) with D = diag(d0, d1) a one-qubit gate on t is the
+//    diagonal |c,t> -> d_{t xor c} |c,t>: D on t, then diag(d1/d0, d0/d1) on t
+//    controlled by c (two phase gates instead of two exchanges and a phase);
+//  * a one-qubit gate is multiplied into the previous one-qubit gate on its
+//    bit when every gate in between that touches the bit commutes with it.
+// Measurements and renormalisations stay where they are and stop folding.
+std::vector<BGate> fold_gates(const std::vector<BGate>& in)
+{
+  std::vector<BGate> a;
+  a.reserve(in.size());
+  for (size_t i = 0; i < in.size(); ++i) {
+    if (i + 2 < in.size()) {
+      const BGate &x = in[i], &d = in[i + 1], &y = in[i + 2];
+      const bool cnot = x.kind == FG_DENSE && x.cbits.size() == 1 && is_exchange(x.m);
+      if (cnot && y.kind == FG_DENSE && y.cbits == x.cbits && y.a == x.a && is_exchange(y.m) && is_1q(d) &&
+          d.a == x.a && is_diag(d.m)) {
+        const double d0r = d.m[0], d0i = d.m[1], d1r = d.m[6], d1i = d.m[7];
+        const double n0 = d0r * d0r + d0i * d0i, n1 = d1r * d1r + d1i * d1i;
+        if (n0 > 0 && n1 > 0) {
+          BGate p = d;
+          set_1q(p, d.m);
+          BGate c;
+          c.kind = FG_PHASE;
+          c.cbits = x.cbits;
+          c.a = x.a;
+          // d1 / d0 and d0 / d1
+          c.m[0] = (d1r * d0r + d1i * d0i) / n0;
+          c.m[1] = (d1i * d0r - d1r * d0i) / n0;
+          c.m[6] = (d0r * d1r + d0i * d1i) / n1;
+          c.m[7] = (d0i * d1r - d0r * d1i) / n1;
+          c.phase_only = is_one(c.m[0], c.m[1]);
+          a.push_back(std::move(p));
+          a.push_back(std::move(c));
+          i += 2;
+          continue;
+        }
+      }
+    }
+    a.push_back(in[i]);
+  }
+  std::vector<BGate> out;
+  out.reserve(a.size());
+  std::unordered_map<int, int> last;                // bit -> index in out of its pending one-qubit gate
+  std::unordered_map<int, std::vector<int>> since;  // bit -> gates touching it after that one
+  for (BGate& g : a) {
+    if (runs_alone(g.kind)) {
+      out.push_back(std::move(g));
+      last.clear();
+      since.clear();
+      continue;
+    }
+    if (is_1q(g)) {
+      const int b = g.a;
+      auto it = last.find(b);
+      if (it != last.end()) {
+        bool ok = true;
+        for (int j : since[b])
+          if (!commutes_on_bit(out[j], b, g.m)) { ok = false; break; }
+        if (ok) {
+          double r[8];
+          mul2(g.m, out[it->second].m, r);
+          set_1q(out[it->second], r);
+          continue;
+        }
+      }
+      out.push_back(std::move(g));
+      last[b] = (int)out.size() - 1;
+      since[b].clear();
+      continue;
+    }
+    out.push_back(std::move(g));
+    const int idx = (int)out.size() - 1;
+    uint64_t bits = gate_bits(out.back());
+    while (bits) {
+      const int b = __builtin_ctzll(bits);
+      bits &= bits - 1;
+      if (last.count(b)) since[b].push_back(idx);
+    }
+  }
+  return out;
+}
+
 // flags: QSV_FLAG_FP64_PASSES keeps complex64 pass arithmetic in FP64; by
 // default it is FP32, renormalised pass by pass (15 % faster per 30-qubit
-// layer: no conversions, same register budget).
-int run_fused(qsv_state* s, const std::vector<BGate>& gs, uint32_t flags)
+// layer: no conversions, same register budget). The gates are folded first.
+int run_fused(qsv_state* s, const std::vector<BGate>& gs_in, uint32_t flags)
 {
+  const std::vector<BGate> gs = fold_gates(gs_in);
   const int TB = s->prec == QSV_C128 ? kTileBitsC128 : kTileBitsC64;
   if (s->n < TB) {
     for (const auto& g : gs) {
@@ -849,15 +1089,15 @@ bool plan_group_passes(int n, int prec, const std::vector<BGate>& gs, std::vecto
   view.n = n;
   view.prec = prec;
   view.S = 1ull << n;
-  const size_t hdr = offsetof(FPass, gates);
   bool ok = true;
+  PProg prog;
   fuse_walk(
       &view, gs, false,
       [&](const FPass& p) {
-        const size_t len = (hdr + (size_t)p.ngates * sizeof(FGate) + 255) & ~(size_t)255;
+        const size_t len = (compile_pass(p, prog) + 255) & ~(size_t)255;
         const size_t off = bytes.size();
         bytes.resize(off + len, 0);
-        std::memcpy(bytes.data() + off, &p, std::min(len, sizeof(FPass)));
+        std::memcpy(bytes.data() + off, &prog, std::min(len, sizeof(PProg)));
         offsets.push_back((int64_t)off);
         return QSV_OK;
       },
@@ -1562,6 +1802,7 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
       tz[k - T0] = bm;
       tc[k - T0] = d->term_coeff[k];
     }
+    if (!dec_rc[i]) decoded[i] = fold_gates(decoded[i]);
     if (!dec_rc[i] && n > lds_n) plans[i].ok = plan_group_passes(n, prec, decoded[i], plans[i].bytes, plans[i].offs);
   });
   for (uint64_t i = 0; i < N; ++i)
@@ -1651,7 +1892,7 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
   }
   // circuits too large for LDS, grouped by qubit count: all instances of a
   // group sit back to back in one arena and advance one pass round per launch
-  // (k_tile group mode, FP64 arithmetic); a program that needs a gate run
+  // (k_pass group mode, FP64 arithmetic); a program that needs a gate run
   // alone takes a pooled state of its own instead
   std::vector<qsv_state*> used;
   std::vector<size_t> singles;
@@ -1724,7 +1965,7 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
         const int64_t* const d_offs = static_cast<const int64_t*>(g_batch.offs.p);
         launch_fill_group(prec, g_group_arena, S, c, gst);
         for (size_t r = 0; r < rounds; ++r)
-          launch_tile_group(prec, g_group_arena, S, (unsigned)c, d_bytes, d_offs + r * c, gst);
+          launch_pass_group(prec, g_group_arena, S, (unsigned)c, d_bytes, d_offs + r * c, gst);
         // energies: one launch for the chunk
         std::vector<uint64_t> meta(3 * c);   // term ranges, then destinations
         for (uint64_t y = 0; y < c; ++y) {
@@ -2497,6 +2738,9 @@ int qsv_server_submit(qsv_server* sv, int num_qubits, const qsv_op* ops, uint64_
   const uint64_t B = sv->cfg.precision == QSV_C128 ? 16 : 8;
   r->bytes = (1ull << num_qubits) * B;
   r->small = sv->cfg.batch_small && num_qubits <= sv->lds_n;
+  // small requests run as batched LDS programs: fold their gates here (large
+  // ones are folded by the fused runner, or run gate by gate without FUSE)
+  if (r->small && (sv->cfg.flags & QSV_FLAG_FUSE)) r->gates = fold_gates(r->gates);
   {
     std::lock_guard<std::mutex> lk(sv->mu);
     if (r->bytes > sv->budget) {

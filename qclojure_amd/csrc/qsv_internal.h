@@ -99,7 +99,7 @@ void launch_outcome_hist(int prec, const void* psi, uint64_t S, int m, const int
 void launch_collapse(int prec, void* psi, uint64_t S, int m, const int* pos, uint64_t sel, double scale,
                      hipStream_t st);
 
-// ---- fused tiled passes (qsv_fuse.hip) ---------------------------------------
+// ---- fused tiled passes: planning (qsv_api.hip) ------------------------------
 constexpr int kTileBitsC128 = 12;   // 4096 x 16 B = 64 KiB of LDS per tile
 constexpr int kTileBitsC64 = 12;    // 4096 x  8 B = 32 KiB: three workgroups per CU
 constexpr int kMaxPassGates = 96;
@@ -155,14 +155,54 @@ struct FPass {
   FGate gates[kMaxPassGates];
 };
 
+// ---- fused passes as micro-op programs (qsv_pass.hip) ------------------------
+// A pass compiled against the layouts it runs in: every op names the register
+// bit, slot mask and thread-id mask it acts on (compile_pass, qsv_api.hip).
+constexpr int kPassTileBits = 12;
+constexpr int kMaxPassOps = kMaxPassGates + kMaxPassStages + 1;
+// Dense structures share the DM_* numbering.
+enum { PM_GENERAL = DM_GENERAL, PM_REAL = DM_REAL, PM_RX = DM_RX, PM_EXCHANGE = DM_EXCHANGE };
+enum {
+  PO_DENSE = 0,       // + 4 * register bit + structure (PM_*): 0..15
+  PO_PHASE = 16,      // + 4 * only1 + register bit: diag(d0, d1) on a register bit, 16..23
+  PO_PHASE_T = 24,    // diag(d0, d1) on a thread bit (aux = its thread-id bit)
+  PO_BLOCKADE = 25,   // phase where exactly one of the bits aux (thread) / kc (slot) is set
+  PO_RELAYOUT = 26,   // new register mask aux; only1 = 1: wave-local round trip
+  PO_SWAP = 27,       // exchange local bits kc and aux where the local bits tmask are set
+};
+struct POp {
+  int32_t code;
+  uint32_t tmask;     // thread-id bits that must be set (thread-bit controls)
+  uint32_t kc;        // slot bits that must be set (register-bit controls)
+  uint32_t aux;
+  int32_t only1;      // PO_PHASE_T: d0 == 1; PO_RELAYOUT: wave-local
+  int32_t pad;
+  double m[8];        // as FGate
+  float f[8];         // m rounded to FP32, for FP32-arithmetic passes
+};
+struct PHead {
+  int32_t nops;
+  int32_t renorm;                   // FP_RENORM_* bits
+  uint32_t load_mask, store_mask;   // register masks of the global load / store
+  int32_t store_local;              // the layout change before the store is wave-local
+  int32_t pad;
+  int32_t pos[16];                  // global index bit of tile-local bit i
+  int32_t sorted_pos[16];           // the same bits in ascending order
+};
+struct PProg {
+  PHead h;
+  POp ops[kMaxPassOps];
+};
 // f32: complex64 arithmetic in FP32 (ignored for complex128, always FP64)
-void launch_tile(int prec, bool f32, void* psi, const FPass* dpass, uint64_t ntiles, const TileNorm& nb,
+void launch_pass(int prec, bool f32, void* psi, const PProg* dprog, uint64_t ntiles, const TileNorm& nb,
                  hipStream_t st);
-void launch_norm_ctl(const TileNorm& nb, uint64_t ntiles, bool first, hipStream_t st);
 // A group of `count` states of S amplitudes (S >= one tile) back to back: one
-// pass round (offs[y] = byte offset of instance y's pass in `passes`, or -1).
-void launch_tile_group(int prec, void* base, uint64_t S, unsigned count, const char* passes, const int64_t* offs,
+// pass round (offs[y] = byte offset of instance y's program in `progs`, or -1).
+void launch_pass_group(int prec, void* base, uint64_t S, unsigned count, const char* progs, const int64_t* offs,
                        hipStream_t st);
+
+// The scale the next pass of a complex64 FP32 run applies (from the per-tile norms).
+void launch_norm_ctl(const TileNorm& nb, uint64_t ntiles, bool first, hipStream_t st);
 void launch_fill_group(int prec, void* base, uint64_t S, uint64_t count, hipStream_t st);
 
 // ---- batched small circuits (qsv_batch.hip) -----------------------------------

@@ -157,3 +157,50 @@ def test_c64_fp32_passes_renormalise(eng):
         for s in (f32, f64):
             c = s.compare(ref)
             assert c["fidelity"] > 1 - 1e-6 and c["max_abs_diff"] < 1e-5, c
+
+
+def _fold_heavy_program(rng, n, layers):
+    """Programs full of what the host folds and of what must stop a fold:
+    RZ RY RZ chains, RZ on a CNOT control and RX on its target (both commute
+    through it), CNOT RZ CNOT and CNOT PHASE CNOT triples, CZ between
+    diagonals, and H on a CNOT target or a SWAP in between (no fold)."""
+    rows = []
+    for _ in range(layers):
+        for q in range(n):
+            for k in ("RZ", "RY", "RZ"):
+                rows.append((OP[k], q, 0, 0, 0, float(rng.uniform(0, 2 * math.pi))))
+        perm = rng.permutation(n)
+        for j in range(0, n - 1, 2):
+            c, t = int(perm[j]), int(perm[j + 1])
+            r = rng.uniform()
+            ang = float(rng.uniform(0, 2 * math.pi))
+            if r < 0.3:
+                rows += [(OP["CNOT"], c, t, 0, 0, 0.0), (OP["RZ"], t, 0, 0, 0, ang), (OP["CNOT"], c, t, 0, 0, 0.0)]
+            elif r < 0.4:
+                rows += [(OP["CNOT"], c, t, 0, 0, 0.0), (OP["PHASE"], t, 0, 0, 0, ang), (OP["CNOT"], c, t, 0, 0, 0.0)]
+            elif r < 0.6:
+                rows += [(OP["CNOT"], c, t, 0, 0, 0.0), (OP["RZ"], c, 0, 0, 0, ang), (OP["RX"], t, 0, 0, 0, ang)]
+            elif r < 0.75:
+                rows += [(OP["CZ"], c, t, 0, 0, 0.0), (OP["T"], c, 0, 0, 0, 0.0), (OP["S"], t, 0, 0, 0, 0.0)]
+            elif r < 0.9:
+                rows += [(OP["CNOT"], c, t, 0, 0, 0.0), (OP["H"], t, 0, 0, 0, 0.0), (OP["H"], t, 0, 0, 0, 0.0)]
+            else:
+                rows += [(OP["SWAP"], c, t, 0, 0, 0.0), (OP["RZ"], c, 0, 0, 0, ang)]
+    return qc.ops_from_tuples(rows)
+
+
+@pytest.mark.parametrize("n", [12, 14, 17])
+def test_folded_programs_vs_oracle(eng, oracle, n):
+    """The fused path folds gates on the host; the result must still be the
+    oracle's state: 1e-12 for complex128, complex64 at its fidelity bound."""
+    rng = np.random.default_rng(900 + n)
+    ops = _fold_heavy_program(rng, n, 6)
+    exp = oracle.run(n, ops)
+    got = _run(eng, n, ops, "c128", QSV_FLAG_FUSE)
+    err = float(np.max(np.abs(got - exp)))
+    note(f"folded n={n} ops={len(ops)} c128 max|d|={err:.2e}")
+    assert err < 1e-12
+    got64 = _run(eng, n, ops, "c64", QSV_FLAG_FUSE)
+    fid = abs(np.vdot(exp, got64)) ** 2
+    note(f"folded n={n} c64 1-F={1 - fid:.2e}")
+    assert abs(1 - fid) < 1e-6
