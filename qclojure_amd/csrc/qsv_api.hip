@@ -560,6 +560,38 @@ inline uint32_t wave_bits(uint32_t reg)
 // Resolve every gate of p against the register layout it runs in: register
 // bits become slot indices, thread bits thread-id bits (thread t holds local
 // bits pdep(t, ~reg)). Returns the byte length of the program.
+// The kernel's LDS swizzle of a tile-local index (k_pass): XOR-linear.
+inline uint32_t pass_swz(uint32_t l) { return l ^ (((l >> 4) ^ (l >> 8)) & 15u); }
+
+// Offsets of layout `reg` for the kernel: LDS bytes of 8-byte elements and
+// global index bits (pos: global bit of each tile-local bit).
+void make_layout(uint32_t reg, const int32_t pos[16], PLayout& L)
+{
+  constexpr uint32_t FULL = (1u << kPassTileBits) - 1u;
+  std::memset(&L, 0, sizeof(L));
+  int j = 0;
+  for (int b = 0; b < kPassTileBits; ++b)
+    if (((FULL & ~reg) >> b) & 1u) {
+      L.lt[j] = pass_swz(1u << b) * 8u;
+      L.gt[j] = 1ull << pos[b];
+      ++j;
+    }
+  int rb[4], nr = 0;
+  for (int b = 0; b < kPassTileBits; ++b)
+    if ((reg >> b) & 1u) rb[nr++] = b;
+  for (int k = 0; k < 16; ++k) {
+    uint32_t l = 0;
+    uint64_t g = 0;
+    for (int i = 0; i < 4; ++i)
+      if ((k >> i) & 1) {
+        l |= 1u << rb[i];
+        g |= 1ull << pos[rb[i]];
+      }
+    L.ls[k] = pass_swz(l) * 8u;
+    L.gs[k] = g;
+  }
+}
+
 size_t compile_pass(const FPass& p, PProg& out)
 {
   constexpr uint32_t FULL = (1u << kPassTileBits) - 1u;
@@ -567,8 +599,11 @@ size_t compile_pass(const FPass& p, PProg& out)
   out.h.renorm = p.renorm;
   out.h.load_mask = p.load_mask;
   out.h.store_mask = p.store_mask;
-  std::memcpy(out.h.pos, p.pos, sizeof(out.h.pos));
   std::memcpy(out.h.sorted_pos, p.sorted_pos, sizeof(out.h.sorted_pos));
+  int nl = 0;
+  make_layout(p.load_mask, p.pos, out.lay[nl]);
+  out.h.load_lay = nl++;
+  int cur = out.h.load_lay;
   int n = 0;
   uint32_t reg = p.load_mask;
   auto emit = [&]() -> POp& {
@@ -579,11 +614,15 @@ size_t compile_pass(const FPass& p, PProg& out)
   for (int s = 0; s < p.nstages; ++s) {
     const FStage& st = p.stages[s];
     if (st.regmask != reg) {
+      make_layout(st.regmask, p.pos, out.lay[nl]);
       POp& op = emit();
       op.code = PO_RELAYOUT;
+      op.tmask = (uint32_t)cur;
+      op.kc = (uint32_t)nl;
       op.aux = st.regmask;
       op.only1 = wave_bits(reg) == wave_bits(st.regmask) ? 1 : 0;
       reg = st.regmask;
+      cur = nl++;
     }
     const uint32_t nonreg = FULL & ~reg;
     auto slot_mask = [&](uint32_t local) {
@@ -635,7 +674,15 @@ size_t compile_pass(const FPass& p, PProg& out)
     }
   }
   out.h.nops = n;
-  out.h.store_local = (reg != p.store_mask && wave_bits(reg) == wave_bits(p.store_mask)) ? 1 : 0;
+  if (reg == p.store_mask) {
+    out.h.store_lay = cur;
+  } else {
+    make_layout(p.store_mask, p.pos, out.lay[nl]);
+    out.h.store_lay = nl++;
+    out.h.store_local = wave_bits(reg) == wave_bits(p.store_mask) ? 1 : 0;
+  }
+  out.h.nlay = nl;
+  out.h.lay_off = (int32_t)offsetof(PProg, lay);
   return offsetof(PProg, ops) + (size_t)n * sizeof(POp);
 }
 
@@ -735,12 +782,30 @@ size_t build_pass(const qsv_state* s, const std::vector<BGate>& gs, const std::v
   std::memset(&p, 0, sizeof(p));
   uint64_t tile = set;
   for (int b = 0; __builtin_popcountll(tile) < TB; ++b) tile |= 1ull << b;
+  // Local bit order: the low run first (coalesced lanes), then the other tile
+  // bits, and at local bits TB-2, TB-1 the two that the fewest dense gates
+  // target. Register masks avoid those two where they can, so the wave-id
+  // bits of the thread index (the top two non-register bits) stay on them
+  // and most layout changes are wave-local.
+  int ntarget[64] = {};
+  for (size_t i = 0; i < chosen.size(); ++i) {
+    const BGate& g = gs[chosen[i]];
+    if (g.kind == FG_DENSE && g.a >= 0) ntarget[g.a]++;
+  }
+  int w1 = -1, w2 = -1;   // the two least-targeted non-low tile bits
+  for (int b = s->n - 1; b >= L; --b) {
+    if (!((tile >> b) & 1ull)) continue;
+    if (w1 < 0 || ntarget[b] < ntarget[w1]) { w2 = w1; w1 = b; }
+    else if (w2 < 0 || ntarget[b] < ntarget[w2]) w2 = b;
+  }
   int loc[64];
   for (int b = 0; b < 64; ++b) loc[b] = -1;
   int nl = 0;
   for (int b = 0; b < L; ++b) { p.pos[nl] = b; loc[b] = nl++; }
   for (int b = L; b < s->n; ++b)
-    if ((tile >> b) & 1ull) { p.pos[nl] = b; loc[b] = nl++; }
+    if (((tile >> b) & 1ull) && b != w1 && b != w2) { p.pos[nl] = b; loc[b] = nl++; }
+  for (int b : {std::min(w1, w2), std::max(w1, w2)})
+    if (b >= 0) { p.pos[nl] = b; loc[b] = nl++; }
   int ns = 0;
   for (int b = 0; b < s->n; ++b)
     if ((tile >> b) & 1ull) p.sorted_pos[ns++] = b;
@@ -776,15 +841,20 @@ size_t build_pass(const qsv_state* s, const std::vector<BGate>& gs, const std::v
   p.stages[nst++] = FStage{cur, g0, (int32_t)taken};
   p.nstages = nst;
   p.ngates = (int)taken;
-  // fill every register mask to RB bits, high non-low tile bits first
+  // fill every register mask to RB bits: non-low bits below the wave pair
+  // (highest first, so consecutive stages tend to share them), then the wave
+  // pair, then low bits
   for (int k = 0; k < nst; ++k) {
     uint32_t r = p.stages[k].regmask;
-    for (int b = TB - 1; b >= 0 && __builtin_popcount(r) < RB; --b)
-      if (b >= L && !((r >> b) & 1u)) r |= 1u << b;
+    for (int b = TB - 3; b >= L && __builtin_popcount(r) < RB; --b) r |= 1u << b;
+    for (int b = TB - 1; b >= TB - 2 && __builtin_popcount(r) < RB; --b) r |= 1u << b;
     for (int b = 0; b < TB && __builtin_popcount(r) < RB; ++b) r |= 1u << b;
     p.stages[k].regmask = r;
   }
-  const uint32_t natural = ((1u << TB) - 1u) & ~((1u << (TB - RB)) - 1u);
+  // load/store layout when a stage's own would split the low run: registers
+  // just below the wave pair, so the lanes cover the low run
+  const uint32_t natural = (L <= TB - 2 - RB) ? (((1u << RB) - 1u) << (TB - 2 - RB))
+                                              : (((1u << TB) - 1u) & ~((1u << (TB - RB)) - 1u));
   p.load_mask = (p.stages[0].regmask & lowmask) ? natural : p.stages[0].regmask;
   p.store_mask = (p.stages[nst - 1].regmask & lowmask) ? natural : p.stages[nst - 1].regmask;
   return taken;
@@ -1094,10 +1164,16 @@ bool plan_group_passes(int n, int prec, const std::vector<BGate>& gs, std::vecto
   fuse_walk(
       &view, gs, false,
       [&](const FPass& p) {
-        const size_t len = (compile_pass(p, prog) + 255) & ~(size_t)255;
+        // header and ops, then the layout table right after the last op
+        const size_t ops_end = compile_pass(p, prog);
+        const size_t lay_bytes = (size_t)prog.h.nlay * sizeof(PLayout);
+        const size_t lay_at = (ops_end + 15) & ~(size_t)15;
+        prog.h.lay_off = (int32_t)lay_at;
+        const size_t len = (lay_at + lay_bytes + 255) & ~(size_t)255;
         const size_t off = bytes.size();
         bytes.resize(off + len, 0);
-        std::memcpy(bytes.data() + off, &prog, std::min(len, sizeof(PProg)));
+        std::memcpy(bytes.data() + off, &prog, ops_end);
+        std::memcpy(bytes.data() + off + lay_at, prog.lay, lay_bytes);
         offsets.push_back((int64_t)off);
         return QSV_OK;
       },

@@ -167,7 +167,7 @@ enum {
   PO_PHASE = 16,      // + 4 * only1 + register bit: diag(d0, d1) on a register bit, 16..23
   PO_PHASE_T = 24,    // diag(d0, d1) on a thread bit (aux = its thread-id bit)
   PO_BLOCKADE = 25,   // phase where exactly one of the bits aux (thread) / kc (slot) is set
-  PO_RELAYOUT = 26,   // new register mask aux; only1 = 1: wave-local round trip
+  PO_RELAYOUT = 26,   // layouts tmask -> kc (PProg::lay), new register mask aux; only1: wave-local
   PO_SWAP = 27,       // exchange local bits kc and aux where the local bits tmask are set
 };
 struct POp {
@@ -180,18 +180,35 @@ struct POp {
   double m[8];        // as FGate
   float f[8];         // m rounded to FP32, for FP32-arithmetic passes
 };
+// A register layout of a tile resolved on the host. Thread t, slot k holds
+// tile-local index pdep(t, ~reg) | pdep(k, reg); with t_j the bits of t, its
+// LDS byte offset is XOR_j (t_j ? lt[j] : 0) XOR ls[k] (the swizzle is
+// XOR-linear) and its global element offset is
+// tile base | OR_j (t_j ? gt[j] : 0) | gs[k].
+struct PLayout {
+  uint32_t lt[8];
+  uint32_t ls[16];
+  uint64_t gt[8];
+  uint64_t gs[16];
+};
+constexpr int kMaxPassLayouts = kMaxPassStages + 2;
 struct PHead {
   int32_t nops;
   int32_t renorm;                   // FP_RENORM_* bits
   uint32_t load_mask, store_mask;   // register masks of the global load / store
   int32_t store_local;              // the layout change before the store is wave-local
+  int32_t load_lay, store_lay;      // their entries in the layout table
+  int32_t nlay;
+  int32_t lay_off;                  // byte offset of the layout table from the program start
   int32_t pad;
-  int32_t pos[16];                  // global index bit of tile-local bit i
-  int32_t sorted_pos[16];           // the same bits in ascending order
+  int32_t sorted_pos[16];           // the tile's global bits in ascending order
 };
+// Ops come right after the header; the layout table sits at h.lay_off, at
+// `lay` here or packed right after the last op (grouped batches).
 struct PProg {
   PHead h;
   POp ops[kMaxPassOps];
+  PLayout lay[kMaxPassLayouts];
 };
 // f32: complex64 arithmetic in FP32 (ignored for complex128, always FP64)
 void launch_pass(int prec, bool f32, void* psi, const PProg* dprog, uint64_t ntiles, const TileNorm& nb,

@@ -257,6 +257,7 @@ __global__ __launch_bounds__(kThreads, sizeof(A) == 4 ? 5 : sizeof(R) == 4 ? 4 :
   const uint32_t tile = blockIdx.x;
   const uint32_t t = threadIdx.x;
   const PHead& h = prog->h;
+  const PLayout* lays = reinterpret_cast<const PLayout*>(reinterpret_cast<const char*>(prog) + h.lay_off);
 
   // tile base: zeros inserted at the tile's global bits (ascending)
   uint64_t base = tile;
@@ -266,43 +267,24 @@ __global__ __launch_bounds__(kThreads, sizeof(A) == 4 ? 5 : sizeof(R) == 4 ? 4 :
     base = ((base ^ lo) << 1) | lo;
   }
 
-  uint32_t reg = h.load_mask;
-  uint32_t tl, q0, q1, q2, q3;   // thread part of the local index; register local bits
-  auto set_layout = [&](uint32_t mask) __attribute__((always_inline)) {
-    tl = pdep32(t, FULL & ~mask);
-    uint32_t m = mask;
-    q0 = m & (~m + 1u); m &= m - 1;
-    q1 = m & (~m + 1u); m &= m - 1;
-    q2 = m & (~m + 1u); m &= m - 1;
-    q3 = m & (~m + 1u);
-  };
-  auto goff = [&](uint32_t l) __attribute__((always_inline)) -> uint64_t {
-    uint64_t g = 0;
+  static_assert(E == 8, "the host resolves LDS offsets for 8-byte elements");
+  uint32_t reg = h.load_mask;   // register mask of the current layout
+  int cur = h.load_lay;         // its descriptor
+  // per-thread parts of a layout's offsets (the slot parts are uniform)
+  auto lds_base = [&](const PLayout& L) __attribute__((always_inline)) -> uint32_t {
+    uint32_t b = 0;
 #pragma unroll
-    for (int i = 0; i < TB; ++i)
-      if (l & (1u << i)) g |= 1ull << h.pos[i];
-    return g;
+    for (int j = 0; j < 8; ++j) b ^= ((t >> j) & 1u) ? L.lt[j] : 0u;
+    return b;
   };
-  // global element offsets of the 16 slots in the current layout
-  auto gaddr = [&](uint64_t (&ga)[kSlots]) {
-    const uint64_t gt = base | goff(tl);
-    const uint64_t g0 = goff(q0), g1 = goff(q1), g2 = goff(q2), g3 = goff(q3);
+  auto gaddr = [&](const PLayout& L, uint64_t (&ga)[kSlots]) __attribute__((always_inline)) {
+    uint64_t g = base;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) g |= ((t >> j) & 1u) ? L.gt[j] : 0ull;
     each_slot([&](auto kk) __attribute__((always_inline)) {
       constexpr int k = decltype(kk)::value;
-      ga[k] = gt | ((k & 1) ? g0 : 0ull) | ((k & 2) ? g1 : 0ull) | ((k & 4) ? g2 : 0ull) | ((k & 8) ? g3 : 0ull);
+      ga[k] = g | L.gs[k];
     });
-  };
-  // LDS byte offsets: swz is XOR-linear and a local index is the XOR of its
-  // thread part and its register part, so slot k's offset is a per-thread base
-  // XOR a workgroup-uniform slot offset.
-  auto roff = [](int k, uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3) __attribute__((always_inline)) -> uint32_t {
-    return ((k & 1) ? a0 : 0u) ^ ((k & 2) ? a1 : 0u) ^ ((k & 4) ? a2 : 0u) ^ ((k & 8) ? a3 : 0u);
-  };
-  auto rsw = [](uint32_t r) __attribute__((always_inline)) -> uint32_t { return swz(r) * E; };
-  auto tbase = [&]() __attribute__((always_inline)) -> uint32_t {
-    uint32_t b = swz(tl) * E;
-    asm("" : "+v"(b));   // keep the scaling by E out of the XOR chain
-    return b;
   };
   auto st_r = [&](uint32_t off, R v) __attribute__((always_inline)) { *reinterpret_cast<R*>(smem + off) = v; };
   auto ld_r = [&](uint32_t off) __attribute__((always_inline)) -> R { return *reinterpret_cast<const R*>(smem + off); };
@@ -316,44 +298,43 @@ __global__ __launch_bounds__(kThreads, sizeof(A) == 4 ? 5 : sizeof(R) == 4 ? 4 :
 
   typename SV<R>::T xr, xi;
   bool local_pending = false;   // a wave-local LDS round trip may still be read by its wave
-  // Move the tile to layout `to` through LDS; wave_local: no workgroup barrier.
-  auto relayout = [&](uint32_t to, bool wave_local) __attribute__((always_inline)) {
+  // Move the tile from layout lay[from] to lay[to] through LDS; wave_local:
+  // no workgroup barrier.
+  auto relayout = [&](int from, int to, bool wave_local) __attribute__((always_inline)) {
     if (!wave_local && local_pending) __syncthreads();
-    const uint32_t fb = tbase();
-    const uint32_t f0 = rsw(q0), f1 = rsw(q1), f2 = rsw(q2), f3 = rsw(q3);
-    set_layout(to);
-    const uint32_t bb = tbase();
-    const uint32_t b0 = rsw(q0), b1 = rsw(q1), b2 = rsw(q2), b3 = rsw(q3);
+    const PLayout& F = lays[from];
+    const PLayout& G = lays[to];
+    const uint32_t fb = lds_base(F), bb = lds_base(G);
     if constexpr (SPLIT) {
       each_slot([&](auto kk) __attribute__((always_inline)) {
         constexpr int k = decltype(kk)::value;
-        st_r(fb ^ roff(k, f0, f1, f2, f3), xr[k]);
+        st_r(fb ^ F.ls[k], xr[k]);
       });
       lds_sync(wave_local);
       each_slot([&](auto kk) __attribute__((always_inline)) {
         constexpr int k = decltype(kk)::value;
-        xr[k] = ld_r(bb ^ roff(k, b0, b1, b2, b3));
+        xr[k] = ld_r(bb ^ G.ls[k]);
       });
       lds_sync(wave_local);
       each_slot([&](auto kk) __attribute__((always_inline)) {
         constexpr int k = decltype(kk)::value;
-        st_r(fb ^ roff(k, f0, f1, f2, f3), xi[k]);
+        st_r(fb ^ F.ls[k], xi[k]);
       });
       lds_sync(wave_local);
       each_slot([&](auto kk) __attribute__((always_inline)) {
         constexpr int k = decltype(kk)::value;
-        xi[k] = ld_r(bb ^ roff(k, b0, b1, b2, b3));
+        xi[k] = ld_r(bb ^ G.ls[k]);
       });
       lds_sync(wave_local);
     } else {
       each_slot([&](auto kk) __attribute__((always_inline)) {
         constexpr int k = decltype(kk)::value;
-        st_c(fb ^ roff(k, f0, f1, f2, f3), xr[k], xi[k]);
+        st_c(fb ^ F.ls[k], xr[k], xi[k]);
       });
       lds_sync(wave_local);
       each_slot([&](auto kk) __attribute__((always_inline)) {
         constexpr int k = decltype(kk)::value;
-        { const T v = ld_c(bb ^ roff(k, b0, b1, b2, b3)); xr[k] = v.x; xi[k] = v.y; }
+        { const T v = ld_c(bb ^ G.ls[k]); xr[k] = v.x; xi[k] = v.y; }
       });
       lds_sync(wave_local);
     }
@@ -367,8 +348,15 @@ __global__ __launch_bounds__(kThreads, sizeof(A) == 4 ? 5 : sizeof(R) == 4 ? 4 :
     const uint32_t tm = op.tmask;
     const uint32_t ba = 1u << op.kc, bb = 1u << op.aux;
     const A pr = coef<A>(op, 0), pi = coef<A>(op, 1);
-    const uint32_t sb = tbase(), dsw = swz(ba ^ bb) * E;
-    const uint32_t a0 = rsw(q0), a1 = rsw(q1), a2 = rsw(q2), a3 = rsw(q3);
+    const PLayout& L = lays[cur];
+    const uint32_t sb = lds_base(L), dsw = swz(ba ^ bb) * E;
+    // local index of each slot, from the register mask (swaps are rare)
+    const uint32_t tl = pdep32(t, FULL & ~reg);
+    uint32_t m = reg;
+    const uint32_t q0 = m & (~m + 1u); m &= m - 1;
+    const uint32_t q1 = m & (~m + 1u); m &= m - 1;
+    const uint32_t q2 = m & (~m + 1u); m &= m - 1;
+    const uint32_t q3 = m & (~m + 1u);
     auto moves = [&](int k) __attribute__((always_inline)) -> bool {
       const uint32_t l = tl | ((k & 1) ? q0 : 0u) | ((k & 2) ? q1 : 0u) | ((k & 4) ? q2 : 0u) | ((k & 8) ? q3 : 0u);
       return ((l & tm) == tm) && (((l & ba) != 0) != ((l & bb) != 0));
@@ -384,43 +372,42 @@ __global__ __launch_bounds__(kThreads, sizeof(A) == 4 ? 5 : sizeof(R) == 4 ? 4 :
     if constexpr (SPLIT) {
       each_slot([&](auto kk) __attribute__((always_inline)) {
         constexpr int k = decltype(kk)::value;
-        st_r(sb ^ roff(k, a0, a1, a2, a3) ^ (moves(k) ? dsw : 0u), xr[k]);
+        st_r(sb ^ L.ls[k] ^ (moves(k) ? dsw : 0u), xr[k]);
       });
       __syncthreads();
       each_slot([&](auto kk) __attribute__((always_inline)) {
         constexpr int k = decltype(kk)::value;
-        xr[k] = ld_r(sb ^ roff(k, a0, a1, a2, a3));
+        xr[k] = ld_r(sb ^ L.ls[k]);
       });
       __syncthreads();
       each_slot([&](auto kk) __attribute__((always_inline)) {
         constexpr int k = decltype(kk)::value;
-        st_r(sb ^ roff(k, a0, a1, a2, a3) ^ (moves(k) ? dsw : 0u), xi[k]);
+        st_r(sb ^ L.ls[k] ^ (moves(k) ? dsw : 0u), xi[k]);
       });
       __syncthreads();
       each_slot([&](auto kk) __attribute__((always_inline)) {
         constexpr int k = decltype(kk)::value;
-        xi[k] = ld_r(sb ^ roff(k, a0, a1, a2, a3));
+        xi[k] = ld_r(sb ^ L.ls[k]);
       });
       __syncthreads();
     } else {
       each_slot([&](auto kk) __attribute__((always_inline)) {
         constexpr int k = decltype(kk)::value;
-        st_c(sb ^ roff(k, a0, a1, a2, a3) ^ (moves(k) ? dsw : 0u), xr[k], xi[k]);
+        st_c(sb ^ L.ls[k] ^ (moves(k) ? dsw : 0u), xr[k], xi[k]);
       });
       __syncthreads();
       each_slot([&](auto kk) __attribute__((always_inline)) {
         constexpr int k = decltype(kk)::value;
-        { const T v = ld_c(sb ^ roff(k, a0, a1, a2, a3)); xr[k] = v.x; xi[k] = v.y; }
+        { const T v = ld_c(sb ^ L.ls[k]); xr[k] = v.x; xi[k] = v.y; }
       });
       __syncthreads();
     }
     local_pending = false;
   };
 
-  set_layout(reg);
   {
     uint64_t ga[kSlots];
-    gaddr(ga);
+    gaddr(lays[cur], ga);
     each_slot([&](auto kk) __attribute__((always_inline)) {
       constexpr int k = decltype(kk)::value;
       const T v = __builtin_nontemporal_load(psi + ga[k]);
@@ -502,17 +489,18 @@ __global__ __launch_bounds__(kThreads, sizeof(A) == 4 ? 5 : sizeof(R) == 4 ? 4 :
         cmul_slot<k>(on ? pr : (A)1, on ? pi : (A)0, xr, xi);
       });
     } else if (code == PO_RELAYOUT) {
-      relayout(op.aux, op.only1 != 0);
+      relayout((int)op.tmask, (int)op.kc, op.only1 != 0);
       reg = op.aux;
+      cur = (int)op.kc;
     } else {
       swap_op(op);
     }
   }
 
-  if (reg != h.store_mask) relayout(h.store_mask, h.store_local != 0);
+  if (cur != h.store_lay) relayout(cur, h.store_lay, h.store_local != 0);
   {
     uint64_t ga[kSlots];
-    gaddr(ga);
+    gaddr(lays[h.store_lay], ga);
     each_slot([&](auto kk) __attribute__((always_inline)) {
       constexpr int k = decltype(kk)::value;
       T v;
