@@ -41,13 +41,41 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
+def _host_info() -> dict:
+    mem = None
+    try:
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemTotal:"):
+                    mem = round(int(line.split()[1]) / 2**20, 1)
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"cpu_model": cpu_model(), "cpus_available": avail, "ram_gib": mem}
+
+
+_QAOA_SAMPLE: dict = {}
+
+
 def cpu_baseline(n: int, seed: int) -> dict:
-    """The oracle's pair-update mode (OpenMP, FP64) on one layer of the same
-    construction at n qubits, timed on this host."""
+    """The CPU leg: the oracle (an FP64 C restatement of the reference's
+    algorithms, oracle/qsv_ref.c) timed on this host, and the oracle as the
+    checker of the batched QAOA energies.
+
+    value: the pair-update mode (the same mathematics as in-place O(2^n)
+    updates, OpenMP on the host cores) on one n-qubit layer of the headline
+    construction, in the headline's unit. Beside it: the literal mode (the
+    reference's own algorithm, gate.clj:323-395: every one-qubit gate as a
+    dense 2^n x 2^n Kronecker product times the state, one thread) on two
+    layers of the 12-qubit, 20-layer random circuit."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     from qclojure_amd import workloads as wl
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     ops = wl.random_layer_ops(n, 1, seed)
     psi = np.zeros(1 << n, dtype=np.complex128)
     psi[0] = 1.0
@@ -57,14 +85,50 @@ def cpu_baseline(n: int, seed: int) -> dict:
     dt = time.perf_counter() - t0
     if rc != 0:
         raise RuntimeError("oracle failed")
-    nbytes = 0
-    for r in ops:
-        nbytes += (16 if int(r["kind"]) == 13 else 32) * S   # CNOT touches S/2, a 1-qubit gate S
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": round(nbytes / dt / 1e9, 2), "unit": "GB/s", "cores": threads, "kind": "port",
-            "sample": f"one {n}-qubit complex128 layer of the same construction ({len(ops)} gates, "
-                      f"{nbytes / 1e9:.0f} GB algorithmic) by the oracle's OpenMP pair-update mode: "
-                      f"{dt:.2f} s on {cpu_model()}"}
+    del psi
+    nbytes = sum((16 if int(r["kind"]) == 13 else 32) * S for r in ops)  # CNOT touches S/2, a 1-qubit gate S
+    out = {"value": round(nbytes / dt / 1e9, 2), "unit": "GB/s", "cores": threads, "kind": "port",
+           "sample": f"one {n}-qubit complex128 layer of the same construction ({len(ops)} gates, "
+                     f"{nbytes / 1e9:.0f} GB algorithmic) by the oracle's OpenMP pair-update mode in {dt:.2f} s",
+           "host": _host_info()}
+    log(f"bench: cpu pair mode {out['value']} GB/s on {threads} threads")
+    # the reference's algorithm, literally, on the 12-qubit random circuit
+    n1, layers1 = 12, 2
+    ops1 = wl.random_layer_ops(n1, 20, 1)[:layers1 * (n1 + n1 // 2)]
+    t0 = time.perf_counter()
+    psi1 = oracle.run(n1, ops1, literal=True)
+    dt1 = time.perf_counter() - t0
+    b1 = sum((16 if int(r["kind"]) == 13 else 32) * (1 << n1) for r in ops1)
+    out["literal_12q"] = {"seconds_per_layer": round(dt1 / layers1, 3), "GB_s": round(b1 / dt1 / 1e9, 6),
+                          "cores": 1, "layers_timed": layers1,
+                          "projected_20_layer_circuit_s": round(dt1 / layers1 * 20, 1),
+                          "norm2_minus_1": float(np.sum(np.abs(psi1) ** 2) - 1.0)}
+    log(f"bench: cpu literal 12q {out['literal_12q']}")
+    # the batched complex64 QAOA energies against the FP64 oracle on a
+    # stratified sample: up to two instances per (qubits, p) cell
+    if _QAOA_SAMPLE:
+        insts, e64 = _QAOA_SAMPLE["insts"], _QAOA_SAMPLE["e64"]
+        picked, cells = [], {}
+        for k, inst in enumerate(insts):
+            key = (inst["n"], inst["p"])
+            if cells.get(key, 0) < 2:
+                cells[key] = cells.get(key, 0) + 1
+                picked.append(k)
+        t0 = time.perf_counter()
+        errs = []
+        for k in picked:
+            inst = insts[k]
+            st = oracle.run(inst["n"], wl.qaoa_ops(inst))
+            ref = 0.0
+            for t in inst["hamiltonian"]:
+                ref += t["coefficient"] * oracle.pauli_expectation(st, inst["n"], t["pauli-string"])
+            errs.append(abs(ref - e64[k]))
+        out["qaoa_oracle_check"] = {"instances": len(picked), "cells": len(cells),
+                                    "max_abs_energy_err_c64_vs_fp64_oracle": float(max(errs)),
+                                    "mean_abs_energy_err": float(np.mean(errs)),
+                                    "oracle_seconds": round(time.perf_counter() - t0, 2)}
+        log(f"bench: qaoa oracle check {out['qaoa_oracle_check']}")
+    return out
 
 
 def position_sweep(engine, n: int, prec: str, reps: int = 3) -> dict:
@@ -114,6 +178,7 @@ def bench_batched_qaoa(engine, count: int, seed: int, plain_every: int) -> dict:
     """4096 MaxCut QAOA instances (n in {8..20} even, p in 1..4), complex64:
     per-instance <H_C> from one batched call vs the plain path (one circuit at a
     time: per-gate kernels, then a Pauli-expectation pass per 8 terms)."""
+    import numpy as np
     from qclojure_amd import workloads as wl
     insts = [wl.qaoa_instance(seed * 100003 + k) for k in range(count)]
     batch = []
@@ -130,6 +195,21 @@ def bench_batched_qaoa(engine, count: int, seed: int, plain_every: int) -> dict:
     e64 = packed.run()
     t_call = time.perf_counter() - t0
     t_batch = t_pack + t_call
+    # the same batch again: its grouped launches replay the graph captured by
+    # the first call (an optimiser loop's steady state), then issued kernel by
+    # kernel for comparison
+    t0 = time.perf_counter()
+    e64_replay = packed.run()
+    t_replay = time.perf_counter() - t0
+    os.environ["QSV_BATCH_GRAPH"] = "0"
+    try:
+        t0 = time.perf_counter()
+        e64_eager = packed.run()
+        t_eager = time.perf_counter() - t0
+    finally:
+        del os.environ["QSV_BATCH_GRAPH"]
+    graph_diff = float(max(np.max(np.abs(e64_replay - e64)), np.max(np.abs(e64_eager - e64))))
+    _QAOA_SAMPLE.update(insts=insts, e64=e64)
     e128 = engine.batch_expect_z(batch, "c128")
     err = float(max(abs(a - b) for a, b in zip(e64, e128)))
     # plain path on every plain_every-th instance
@@ -154,6 +234,8 @@ def bench_batched_qaoa(engine, count: int, seed: int, plain_every: int) -> dict:
     sizes = sorted({b[0] for b in batch})
     return {"instances": count, "qubits": sizes, "gates": int(sum(len(b[1]) for b in batch)),
             "batched_s": round(t_batch, 4), "pack_s": round(t_pack, 4), "library_call_s": round(t_call, 4),
+            "library_call_graph_replay_s": round(t_replay, 4), "library_call_eager_launches_s": round(t_eager, 4),
+            "max_abs_diff_replay_eager_vs_first": graph_diff,
             "batched_amp_updates_per_s": round(upd / t_batch, 1),
             "plain_sample_instances": len(sample), "plain_s": round(t_plain, 4),
             "plain_amp_updates_per_s": round(s_upd / t_plain, 1),
@@ -162,12 +244,42 @@ def bench_batched_qaoa(engine, count: int, seed: int, plain_every: int) -> dict:
             "note": "plain = one circuit at a time, per-gate kernels + Pauli reductions; timed on a stratified sample"}
 
 
+def _sequential_real(engine, reqs, arr, flags) -> dict:
+    """One request at a time in arrival order, with real arrivals: a request
+    that arrives while another runs waits in a FIFO queue; latency = results
+    on the host - arrival. flags 0: per-gate kernels; QSV_FLAG_FUSE: fused
+    passes. Then the shots, as the served path takes them."""
+    import numpy as np
+    states = {}
+    lat = []
+    t_start = time.perf_counter()
+    for (n, ops, shots, sd), a in zip(reqs, arr):
+        now = time.perf_counter() - t_start
+        if now < a:
+            time.sleep(a - now)
+        if n not in states:
+            states[n] = engine.DeviceState(n, "c64")
+        s = states[n]
+        s.init_basis(0)
+        s.apply(ops, flags)
+        s.sample(shots, sd)
+        lat.append(time.perf_counter() - t_start - a)
+    t_total = time.perf_counter() - t_start
+    for s in states.values():
+        s.destroy()
+    return {"circuits_per_s": round(len(reqs) / t_total, 2),
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 1),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 1)}
+
+
 def bench_serving(engine, rates, n_req: int, n_plain: int, seed: int) -> dict:
     """Poisson request stream (n log-uniform in [10, 30], depth 10..100 layers,
-    1000 shots, complex64): the native serving loop vs the plain sequential path.
-    The request contents are fixed; each rate rescales the same unit-rate
-    arrival times. Plain latencies come from measured per-request service
-    times through a single FIFO server."""
+    1000 shots, complex64): the native serving loop against two one-at-a-time
+    baselines with the same real arrivals: "plain" (per-gate kernels, the
+    plain GPU path) and "sequential_fused" (fused passes, no batching or
+    overlap), so that the ratio served/sequential_fused is what continuous
+    batching earns on top of fusion. The request contents are fixed; each rate
+    rescales the same unit-rate arrival times."""
     import threading
     import numpy as np
     from qclojure_amd import workloads as wl
@@ -177,31 +289,13 @@ def bench_serving(engine, rates, n_req: int, n_plain: int, seed: int) -> dict:
     unit_arrivals = [r["arrival-s"] for r in stream]
     gates = sum(len(r[1]) for r in reqs)
     n_plain = n_plain if 0 < n_plain < n_req else n_req
-    # plain: one request at a time, per-gate kernels, then shots
-    service = []
-    states = {}
-    for (n, ops, shots, sd) in reqs[:n_plain]:
-        t0 = time.perf_counter()
-        if n not in states:
-            states[n] = engine.DeviceState(n, "c64")
-        s = states[n]
-        s.init_basis(0)
-        s.apply(ops)
-        s.sample(shots, sd)
-        service.append(time.perf_counter() - t0)
-    for s in states.values():
-        s.destroy()
+    # warm every state size and kernel once, untimed
+    _sequential_real(engine, reqs[:4], [0.0] * 4, QSV_FLAG_FUSE)
     out = {"requests": n_req, "gates": gates, "plain_requests": n_plain, "rates": []}
     for lam in rates:
         arr = [a / lam for a in unit_arrivals]
-        # plain FIFO queue over the measured service times
-        done, lat = 0.0, []
-        for a, sv in zip(arr[:n_plain], service):
-            done = max(a, done) + sv
-            lat.append(done - a)
-        plain = {"circuits_per_s": round(n_plain / (done - arr[0]), 2),
-                 "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 1),
-                 "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 1)}
+        plain = _sequential_real(engine, reqs[:n_plain], arr[:n_plain], 0)
+        seqf = _sequential_real(engine, reqs, arr, QSV_FLAG_FUSE)
         # served: real arrivals into the native loop
         with engine.Server("c64", flags=QSV_FLAG_FUSE) as srv:
             ids = [None] * n_req
@@ -216,12 +310,13 @@ def bench_serving(engine, rates, n_req: int, n_plain: int, seed: int) -> dict:
 
             th = threading.Thread(target=submitter)
             th.start()
-            lats = []
+            lats, drift = [], 0.0
             for k in range(n_req):
                 while ids[k] is None:
                     time.sleep(0.0005)
                 info, _ = srv.wait(ids[k], reqs[k][2])
                 lats.append(info["latency-ms"])
+                drift = max(drift, info.get("norm-drift", 0.0))
             th.join()
             t_total = time.perf_counter() - t_start
             st = srv.stats()
@@ -229,10 +324,11 @@ def bench_serving(engine, rates, n_req: int, n_plain: int, seed: int) -> dict:
                   "p50_ms": round(float(np.percentile(lats, 50)), 1),
                   "p99_ms": round(float(np.percentile(lats, 99)), 1),
                   "small_batched": int(st["small_batched_requests"]), "batches": int(st["small_batches"]),
-                  "max_queue_depth": int(st["max_queue_depth"])}
-        out["rates"].append({"lambda_per_s": lam, "served": served, "plain": plain,
-                             "throughput_ratio": round(served["circuits_per_s"] / plain["circuits_per_s"], 2)})
-        log(f"bench: serving lambda={lam}: served {served} plain {plain}")
+                  "max_queue_depth": int(st["max_queue_depth"]), "max_norm_drift": drift}
+        out["rates"].append({"lambda_per_s": lam, "served": served, "plain": plain, "sequential_fused": seqf,
+                             "throughput_ratio": round(served["circuits_per_s"] / plain["circuits_per_s"], 2),
+                             "batching_ratio": round(served["circuits_per_s"] / seqf["circuits_per_s"], 2)})
+        log(f"bench: serving lambda={lam}: served {served} plain {plain} sequential_fused {seqf}")
     return out
 
 
@@ -381,55 +477,19 @@ def bench_la(engine, n: int, seed: int) -> dict:
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--qubits", type=int, default=30)
-    ap.add_argument("--prec", default="c128", choices=["c128", "c64"])
-    ap.add_argument("--seed", type=int, default=2026)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-fused", action="store_true", help="skip the fused tiled-pass measurement")
-    ap.add_argument("--cpu-qubits", type=int, default=28)
-    ap.add_argument("--sweep", metavar="PATH", help="also write GB/s per gate type x bit position to PATH")
-    ap.add_argument("--qaoa", type=int, default=4096, help="batched QAOA instances (0 to skip)")
-    ap.add_argument("--serve", type=int, default=120, help="served requests per arrival rate (0 to skip)")
-    ap.add_argument("--serve-plain", type=int, default=0,
-                    help="requests timed on the plain path (0: the same requests as served)")
-    ap.add_argument("--rates", default="50,200,1000,2000")
-    ap.add_argument("--precision-qubits", type=int, default=32, help="0 to skip the precision study")
-    ap.add_argument("--precision-layers", type=int, default=50)
-    ap.add_argument("--la", type=int, default=4096, help="GEMM size of the linear-algebra section (0 to skip)")
-    ap.add_argument("--calibrate-copy", action="store_true",
-                    help="also copy the state once with the library's copy kernel (a kernel of known bytes for "
-                         "scaling hardware byte counters)")
-    args = ap.parse_args()
-
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the state-vector path "
-              "here runs on a single MI355X; sharding is described in DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
-
+def measure_layers(engine, n: int, prec: str, ops, args, tuned=None) -> dict:
+    """The random-layer circuit at n qubits and one precision: `steps` layers
+    through the per-gate kernels (one kernel per gate, HIP-event timing per
+    kernel class), then the same layers as one fused program."""
     import numpy as np
-    from qclojure_amd import engine, workloads as wl
-    engine.init(0)
-    info = engine.device_info()
-    log(f"bench: device {info['name']} {info['arch']} CUs={info['compute-units']} "
-        f"HBM={info['hbm-bytes'] / 2**30:.0f} GiB")
-
-    n, layers_total = args.qubits, 100
-    ops = wl.random_layer_ops(n, layers_total, args.seed)
+    layers_total = 100
     per = n + n // 2
     layer = lambda k: ops[(k % layers_total) * per:((k % layers_total) + 1) * per]
-
-    dev = engine.DeviceState(n, args.prec)
-    log(f"bench: state {n} qubits {args.prec} ({dev.nbytes / 2**30:.1f} GiB) allocated")
+    dev = engine.DeviceState(n, prec)
+    log(f"bench: state {n} qubits {prec} ({dev.nbytes / 2**30:.1f} GiB) allocated")
     for w in range(args.warmup):
         dev.apply(layer(w))
     dev.sync()
-    log(f"bench: {args.warmup} warmup layers done")
-
     dev.timing(True)
     dev.timing_read(reset=True)
     t0 = time.perf_counter()
@@ -440,10 +500,8 @@ def main():
     stats = dev.timing_read(reset=True)
     dev.timing(False)
     norm = dev.norm2()
-    log(f"bench: {args.steps} layers in {elapsed:.3f} s, norm^2-1 = {norm - 1:.2e}")
-
+    log(f"bench: {prec} {args.steps} layers in {elapsed:.3f} s, norm^2-1 = {norm - 1:.2e}")
     total_bytes = sum(v["bytes"] for v in stats.values())
-    value = total_bytes / elapsed / 1e9
     kern = {}
     for name, v in stats.items():
         if v["launches"]:
@@ -451,18 +509,16 @@ def main():
                           "GB_s": round(v["bytes"] / (v["total_ms"] / 1e3) / 1e9, 1)}
     dom = max(stats, key=lambda k: stats[k]["total_ms"])
     d = stats[dom]
-    achieved = d["bytes"] / (d["total_ms"] / 1e3) / 1e9
     if args.calibrate_copy:
-        other = engine.DeviceState(n, args.prec)
+        other = engine.DeviceState(n, prec)
         other.copy_from(dev)
         other.destroy()
         log(f"bench: calibration copy of {dev.nbytes} B done")
     fused = None
     if not args.no_fused:
         from qclojure_amd._abi import QSV_FLAG_FUSE
-        tuned = None
-        if "QSV_TILE_LOWBITS" not in os.environ:
-            tuned = engine.tune_fused(args.prec)
+        if tuned is None and "QSV_TILE_LOWBITS" not in os.environ:
+            tuned = engine.tune_fused(prec)
             log(f"bench: tuned tile low bits {tuned}")
         # the fused path takes the timed layers as one program, as a circuit is
         # submitted, so passes may span layer boundaries
@@ -484,22 +540,87 @@ def main():
                  "passes_per_layer": round(blk["launches"] / args.steps, 2),
                  "tile_kernel_avg_ms": round(blk["total_ms"] / max(1, blk["launches"]), 4),
                  "tile_kernel_GB_s": round(blk["bytes"] / (blk["total_ms"] / 1e3) / 1e9, 1) if blk["launches"] else None,
+                 "tile_kernel_frac_of_peak": round(blk["bytes"] / (blk["total_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
+                 if blk["launches"] else None,
                  "effective_GB_s": round(total_bytes / args.steps / (f_elapsed / args.steps) / 1e9, 1),
                  "tile_low_bits": tuned["low_bits"] if tuned else int(os.environ["QSV_TILE_LOWBITS"]),
                  "tuner_ms_per_26q_layer": {str(k): round(v, 3) for k, v in tuned["ms_per_layer"].items()} if tuned else None,
                  "note": "effective_GB_s = per-gate algorithmic bytes of the layer / fused layer time"}
-        log(f"bench: fused {fused}")
+        log(f"bench: {prec} fused {fused}")
+    dev.destroy()
+    achieved = d["bytes"] / (d["total_ms"] / 1e3) / 1e9
+    summary = {"precision": prec, "value_GB_s": round(total_bytes / elapsed / 1e9, 1),
+               "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+               "hbm_frac_of_peak": round(total_bytes / elapsed / 1e9 / HBM_PEAK_GBS, 4),
+               "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(achieved / HBM_PEAK_GBS, 4), "kernel": f"k_{dom}",
+                            "avg_launch_ms": round(d["total_ms"] / d["launches"], 4)},
+               "kernels": kern, "fused": fused}
+    return {"elapsed": elapsed, "stats": stats, "total_bytes": total_bytes, "dom": dom, "kernels": kern,
+            "fused": fused, "tuned": tuned, "summary": summary}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--prec", default="c128", choices=["c128", "c64"])
+    ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fused", action="store_true", help="skip the fused tiled-pass measurement")
+    ap.add_argument("--cpu-qubits", type=int, default=30)
+    ap.add_argument("--sweep", metavar="PATH", help="also write GB/s per gate type x bit position to PATH")
+    ap.add_argument("--qaoa", type=int, default=4096, help="batched QAOA instances (0 to skip)")
+    ap.add_argument("--serve", type=int, default=120, help="served requests per arrival rate (0 to skip)")
+    ap.add_argument("--serve-plain", type=int, default=0,
+                    help="requests timed on the plain path (0: the same requests as served)")
+    ap.add_argument("--rates", default="50,200,1000,2000")
+    ap.add_argument("--precision-qubits", type=int, default=32, help="0 to skip the precision study")
+    ap.add_argument("--precision-layers", type=int, default=50)
+    ap.add_argument("--la", type=int, default=4096, help="GEMM size of the linear-algebra section (0 to skip)")
+    ap.add_argument("--no-other-precision", action="store_true",
+                    help="skip the same measurement at the other precision (c64 beside c128)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also copy the state once with the library's copy kernel (a kernel of known bytes for "
+                         "scaling hardware byte counters)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the state-vector path "
+              "here runs on a single MI355X; sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from qclojure_amd import engine, workloads as wl
+    engine.init(0)
+    info = engine.device_info()
+    log(f"bench: device {info['name']} {info['arch']} CUs={info['compute-units']} "
+        f"HBM={info['hbm-bytes'] / 2**30:.0f} GiB")
+
+    n, layers_total = args.qubits, 100
+    ops = wl.random_layer_ops(n, layers_total, args.seed)
+    main_run = measure_layers(engine, n, args.prec, ops, args)
+    other = "c64" if args.prec == "c128" else "c128"
+    other_run = None
+    if not args.no_other_precision:
+        other_run = measure_layers(engine, n, other, ops, args, tuned=main_run.get("tuned"))
+        other_run.pop("tuned", None)
+    elapsed, stats, total_bytes = main_run["elapsed"], main_run["stats"], main_run["total_bytes"]
+    value = total_bytes / elapsed / 1e9
+    dom = main_run["dom"]
+    d = stats[dom]
+    achieved = d["bytes"] / (d["total_ms"] / 1e3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         try:
             with open(tpath) as f:
                 tr = json.load(f)
-            key = f"{dom}:{args.prec}:{n}"
-            traffic = tr.get(key)
+            traffic = tr.get(f"{dom}:{args.prec}:{n}")
         except (OSError, ValueError):
             traffic = None
-    dev.destroy()
 
     out = {
         "metric": METRIC,
@@ -522,9 +643,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": f"k_{dom}", "avg_launch_ms": round(d["total_ms"] / d["launches"], 4)},
-        "kernels": kern,
-        "fused": fused,
+        "kernels": main_run["kernels"],
+        "fused": main_run["fused"],
     }
+    if other_run is not None:
+        out[other] = other_run["summary"]
     def section(name, fn):
         try:
             log(f"bench: {name} ...")

@@ -1164,8 +1164,10 @@ bool plan_group_passes(int n, int prec, const std::vector<BGate>& gs, std::vecto
   fuse_walk(
       &view, gs, false,
       [&](const FPass& p) {
-        // header and ops, then the layout table right after the last op
+        // header and ops, then the layout table right after the last op; the
+        // instance's first pass writes |0..0> itself
         const size_t ops_end = compile_pass(p, prog);
+        if (offsets.empty()) prog.h.renorm |= FP_INIT_ZERO;
         const size_t lay_bytes = (size_t)prog.h.nlay * sizeof(PLayout);
         const size_t lay_at = (ops_end + 15) & ~(size_t)15;
         prog.h.lay_off = (int32_t)lay_at;
@@ -1674,27 +1676,6 @@ constexpr uint64_t kGroupArenaBytes = 8ull << 30;
 void* g_group_arena = nullptr;
 uint64_t g_group_cap = 0;
 
-// Grow-only scratch of a grouped batch (the stream using it is idle).
-int grow_group_scratch(uint64_t** meta, size_t* meta_cap, size_t meta_bytes, double** part, size_t* part_cap,
-                       size_t part_bytes)
-{
-  if (*meta_cap < meta_bytes) {
-    hipFree(*meta);
-    *meta = nullptr;
-    *meta_cap = 0;
-    HIPCHK(hipMalloc((void**)meta, meta_bytes));
-    *meta_cap = meta_bytes;
-  }
-  if (*part_cap < part_bytes) {
-    hipFree(*part);
-    *part = nullptr;
-    *part_cap = 0;
-    HIPCHK(hipMalloc((void**)part, part_bytes));
-    *part_cap = part_bytes;
-  }
-  return QSV_OK;
-}
-
 int grow_group_arena(uint64_t bytes)
 {
   if (bytes <= g_group_cap) return QSV_OK;
@@ -1739,12 +1720,111 @@ struct GrowBuf {
   size_t cap = 0;
   bool pinned = false;
 };
+// One instantiated launch sequence of the grouped passes, found again by the
+// launch parameters it was captured with.
+struct GroupGraph {
+  std::vector<uint64_t> key;
+  hipGraphExec_t exec = nullptr;
+  uint64_t used = 0;
+};
 struct BatchCache {
   hipStream_t st = nullptr, gst = nullptr;
   GrowBuf recs_h{nullptr, 0, true};   // pinned staging of the gate records
-  GrowBuf recs, insts, tz, tc, order, energy, bytes, offs;
+  GrowBuf bytes_h{nullptr, 0, true};  // pinned staging of the grouped pass programs
+  GrowBuf recs, insts, tz, tc, order, energy, bytes, offs, meta, part;
+  std::vector<GroupGraph> graphs;   // at most kGroupGraphs, least recently used replaced
+  uint64_t tick = 0;
 };
+constexpr size_t kGroupGraphs = 4;
 BatchCache g_batch;
+
+// One chunk of a grouped batch: c states of S amplitudes back to back in the
+// arena, `rounds` pass launches, then one energy reduction.
+struct GroupChunk {
+  uint64_t S = 0, c = 0, rounds = 0;
+  uint64_t table_at = 0;   // first entry of the chunk's rounds x c program offsets
+  uint64_t meta_at = 0;    // first entry of its term ranges and destinations
+  uint64_t bps = 1;        // reduction blocks per state
+  uint32_t fill = 0;       // some instance has no pass: fill the arena first
+};
+// Device buffers that every launch of a grouped batch reads or writes.
+struct GroupLaunch {
+  int prec = 0;
+  void* arena = nullptr;
+  const char* progs = nullptr;
+  const int64_t* table = nullptr;
+  const uint64_t* meta = nullptr;
+  double* part = nullptr;
+  const uint64_t* tz = nullptr;
+  const double* tc = nullptr;
+  double* energy = nullptr;
+};
+
+void issue_group_launches(const GroupLaunch& g, const std::vector<GroupChunk>& chunks, hipStream_t st)
+{
+  for (const GroupChunk& ch : chunks) {
+    if (ch.fill) launch_fill_group(g.prec, g.arena, ch.S, ch.c, st);
+    for (uint64_t r = 0; r < ch.rounds; ++r)
+      launch_pass_group(g.prec, g.arena, ch.S, (unsigned)ch.c, g.progs, g.table + ch.table_at + r * ch.c, st);
+    launch_zexpect_group(g.prec, g.arena, ch.S, (unsigned)ch.c, g.tz, g.tc, g.meta + ch.meta_at, g.part,
+                         (unsigned)ch.bps, g.meta + ch.meta_at + 2 * ch.c, g.energy, st);
+  }
+}
+
+// Runs the launch sequence of a grouped batch on st. By default the sequence
+// is captured once into a HIP graph (kernel nodes only) and replayed whenever
+// a later call has the same launches and buffers: an optimiser loop that only
+// changes angles resubmits the whole batch as one graph launch, its programs
+// rewritten in place. QSV_BATCH_GRAPH=0 issues the kernels one by one.
+int run_group_launches(const GroupLaunch& g, const std::vector<GroupChunk>& chunks, hipStream_t st)
+{
+  const char* env = std::getenv("QSV_BATCH_GRAPH");
+  if (env && env[0] == '0') {
+    issue_group_launches(g, chunks, st);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? QSV_OK : fail(QSV_E_DEVICE, std::string("batch: launch failed: ") + hipGetErrorString(e));
+  }
+  std::vector<uint64_t> key = {(uint64_t)g.prec,          (uint64_t)(uintptr_t)g.arena, (uint64_t)(uintptr_t)g.progs,
+                               (uint64_t)(uintptr_t)g.table, (uint64_t)(uintptr_t)g.meta,  (uint64_t)(uintptr_t)g.part,
+                               (uint64_t)(uintptr_t)g.tz,    (uint64_t)(uintptr_t)g.tc,    (uint64_t)(uintptr_t)g.energy};
+  for (const GroupChunk& ch : chunks)
+    key.insert(key.end(), {ch.S, ch.c, ch.rounds, ch.table_at, ch.meta_at, ch.bps, (uint64_t)ch.fill});
+  GroupGraph* hit = nullptr;
+  for (GroupGraph& gg : g_batch.graphs)
+    if (gg.exec && gg.key == key) hit = &gg;
+  if (!hit) {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      return fail(QSV_E_DEVICE, "batch: stream capture failed to start");
+    issue_group_launches(g, chunks, st);
+    const hipError_t le = hipGetLastError();
+    const hipError_t ce = hipStreamEndCapture(st, &graph);
+    hipError_t ie = hipSuccess;
+    if (le == hipSuccess && ce == hipSuccess) ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (graph) hipGraphDestroy(graph);
+    if (le != hipSuccess || ce != hipSuccess || ie != hipSuccess) {
+      if (exec) hipGraphExecDestroy(exec);
+      const hipError_t e = le != hipSuccess ? le : ce != hipSuccess ? ce : ie;
+      return fail(QSV_E_DEVICE, std::string("batch: graph capture failed: ") + hipGetErrorString(e));
+    }
+    if (g_batch.graphs.size() < kGroupGraphs) {
+      g_batch.graphs.emplace_back();
+      hit = &g_batch.graphs.back();
+    } else {
+      hit = &g_batch.graphs[0];
+      for (GroupGraph& gg : g_batch.graphs)
+        if (gg.used < hit->used) hit = &gg;
+      // the stream is idle at a call's start: the old graph has finished
+      if (hit->exec) hipGraphExecDestroy(hit->exec);
+    }
+    hit->key = std::move(key);
+    hit->exec = exec;
+  }
+  hit->used = ++g_batch.tick;
+  if (hipGraphLaunch(hit->exec, st) != hipSuccess) return fail(QSV_E_DEVICE, "batch: graph launch failed");
+  return QSV_OK;
+}
 
 int grow(GrowBuf& b, size_t bytes)
 {
@@ -1771,8 +1851,11 @@ int upload(GrowBuf& b, const T* src, size_t n, hipStream_t st)
 
 void batch_cache_release()
 {
-  for (GrowBuf* b : {&g_batch.recs_h, &g_batch.recs, &g_batch.insts, &g_batch.tz, &g_batch.tc, &g_batch.order,
-                     &g_batch.energy, &g_batch.bytes, &g_batch.offs}) {
+  for (GroupGraph& gg : g_batch.graphs)
+    if (gg.exec) hipGraphExecDestroy(gg.exec);
+  g_batch.graphs.clear();
+  for (GrowBuf* b : {&g_batch.recs_h, &g_batch.bytes_h, &g_batch.recs, &g_batch.insts, &g_batch.tz, &g_batch.tc,
+                     &g_batch.order, &g_batch.energy, &g_batch.bytes, &g_batch.offs, &g_batch.meta, &g_batch.part}) {
     if (b->p) (void)(b->pinned ? hipHostFree(b->p) : hipFree(b->p));
     b->p = nullptr;
     b->cap = 0;
@@ -1842,16 +1925,17 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
   std::vector<uint64_t> rec_off(N + 1, 0);
   const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
   const unsigned nt = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(hw, N / 8 + 1));
-  auto parallel = [&](auto&& body) {
+  auto parallel_over = [&](uint64_t count, auto&& body) {
     std::atomic<uint64_t> next{0};
     auto worker = [&]() {
-      for (uint64_t i; (i = next.fetch_add(1)) < N;) body(i);
+      for (uint64_t i; (i = next.fetch_add(1)) < count;) body(i);
     };
     std::vector<std::thread> pool;
     for (unsigned t = 1; t < nt; ++t) pool.emplace_back(worker);
     worker();
     for (auto& t : pool) t.join();
   };
+  auto parallel = [&](auto&& body) { parallel_over(N, body); };
   parallel([&](uint64_t i) {
     const int n = d->num_qubits[i];
     uint64_t mk = 0;
@@ -1975,39 +2059,31 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
   {
     std::map<int, std::vector<size_t>, std::greater<int>> by_n;   // n (largest first) -> positions in big_idx
     for (size_t k = 0; k < big_idx.size(); ++k) by_n[insts[big_idx[k]].n].push_back(k);
-    hipStream_t gst = nullptr;
-    double* d_part = nullptr;
-    uint64_t* d_meta = nullptr;
-    size_t part_cap = 0, meta_cap = 0;
-    auto gclean = [&]() {
-      if (gst) hipStreamSynchronize(gst);
-      hipFree(d_part);
-      hipFree(d_meta);
-    };
-    if (!by_n.empty()) {
-      if ((!g_batch.gst && hipStreamCreateWithFlags(&g_batch.gst, hipStreamNonBlocking) != hipSuccess) ||
-          hipStreamWaitEvent(gst = g_batch.gst, uploaded, 0) != hipSuccess) {
-        gclean();
-        cleanup();
-        return fail(QSV_E_DEVICE, "batch: group stream setup failed");
-      }
-    }
+    // Plan every chunk first. The descriptors of all chunks go up in one copy
+    // each, so the launches of the whole batch follow one another with no host
+    // wait, and the sequence can be replayed as one graph.
+    std::vector<GroupChunk> chunks;
+    std::vector<std::pair<const Plan*, uint64_t>> copies;   // program bytes and where they go
+    uint64_t n_bytes = 0;
+    std::vector<int64_t> table;
+    std::vector<uint64_t> meta;   // per chunk: term ranges (2c), then destinations (c)
+    uint64_t arena_need = 0;
+    size_t part_need = 0;
     for (auto& kv : by_n) {
       const int n = kv.first;
       const uint64_t S = 1ull << n;
       const uint64_t per_chunk = std::max<uint64_t>(1, std::min<uint64_t>(65535, kGroupArenaBytes / (S * B)));
       std::vector<size_t>& members = kv.second;
       for (size_t c0 = 0; c0 < members.size(); c0 += per_chunk) {
-        // plan every instance of the chunk
-        std::vector<char> bytes;
         std::vector<std::vector<int64_t>> offs_of;
         std::vector<size_t> chunk;
         for (size_t m = c0; m < members.size() && m < c0 + per_chunk; ++m) {
           Plan& pl = big_plans[members[m]];
           if (pl.ok) {
             // per-instance descriptors are 256-byte multiples: appending keeps them aligned
-            const int64_t base = (int64_t)bytes.size();
-            bytes.insert(bytes.end(), pl.bytes.begin(), pl.bytes.end());
+            const int64_t base = (int64_t)n_bytes;
+            copies.emplace_back(&pl, n_bytes);
+            n_bytes += pl.bytes.size();
             std::vector<int64_t> offs(pl.offs);
             for (int64_t& o : offs) o += base;
             chunk.push_back(members[m]);
@@ -2017,58 +2093,81 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
           }
         }
         if (chunk.empty()) continue;
-        const uint64_t c = chunk.size();
-        size_t rounds = 0;
-        for (const auto& o : offs_of) rounds = std::max(rounds, o.size());
-        std::vector<int64_t> table(std::max<size_t>(1, rounds * c), -1);
-        for (uint64_t y = 0; y < c; ++y)
-          for (size_t r = 0; r < offs_of[y].size(); ++r) table[r * c + y] = offs_of[y][r];
-        // the previous chunk's launches read the arena and the old
-        // descriptors: wait, then grow / replace them
-        hipStreamSynchronize(gst);
-        if ((rc = grow_group_arena(c * S * B))) {
-          gclean();
-          cleanup();
-          return rc;
+        GroupChunk ch{};
+        ch.S = S;
+        ch.c = chunk.size();
+        for (const auto& o : offs_of) {
+          ch.rounds = std::max<uint64_t>(ch.rounds, o.size());
+          // every instance's first pass starts from |0..0>; an instance with
+          // no pass at all needs the arena filled
+          ch.fill |= o.empty() ? 1u : 0u;
         }
-        if (upload(g_batch.bytes, bytes.data(), bytes.size(), gst) ||
-            upload(g_batch.offs, table.data(), table.size(), gst)) {
-          gclean();
-          cleanup();
-          return fail(QSV_E_NOMEM, "batch: group descriptor upload failed");
-        }
-        const char* const d_bytes = static_cast<const char*>(g_batch.bytes.p);
-        const int64_t* const d_offs = static_cast<const int64_t*>(g_batch.offs.p);
-        launch_fill_group(prec, g_group_arena, S, c, gst);
-        for (size_t r = 0; r < rounds; ++r)
-          launch_pass_group(prec, g_group_arena, S, (unsigned)c, d_bytes, d_offs + r * c, gst);
-        // energies: one launch for the chunk
-        std::vector<uint64_t> meta(3 * c);   // term ranges, then destinations
-        for (uint64_t y = 0; y < c; ++y) {
+        ch.table_at = table.size();
+        table.resize(table.size() + std::max<uint64_t>(1, ch.rounds * ch.c), -1);
+        for (uint64_t y = 0; y < ch.c; ++y)
+          for (size_t r = 0; r < offs_of[y].size(); ++r) table[ch.table_at + r * ch.c + y] = offs_of[y][r];
+        ch.meta_at = meta.size();
+        meta.resize(meta.size() + 3 * ch.c);
+        for (uint64_t y = 0; y < ch.c; ++y) {
           const uint64_t i = big_idx[chunk[y]];
-          meta[2 * y] = insts[i].t0;
-          meta[2 * y + 1] = insts[i].t1;
-          meta[2 * c + y] = i;
+          meta[ch.meta_at + 2 * y] = insts[i].t0;
+          meta[ch.meta_at + 2 * y + 1] = insts[i].t1;
+          meta[ch.meta_at + 2 * ch.c + y] = i;
         }
-        const unsigned bps = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(64, S / 4096));
-        if ((rc = grow_group_scratch(&d_meta, &meta_cap, meta.size() * sizeof(uint64_t), &d_part, &part_cap,
-                                     (size_t)c * bps * sizeof(double))) ||
-            hipMemcpyAsync(d_meta, meta.data(), meta.size() * sizeof(uint64_t), hipMemcpyHostToDevice, gst) !=
-                hipSuccess) {
-          gclean();
-          cleanup();
-          return rc ? rc : fail(QSV_E_DEVICE, "batch: energy metadata upload failed");
-        }
-        launch_zexpect_group(prec, g_group_arena, S, (unsigned)c, d_tz, d_tc, d_meta, d_part, bps, d_meta + 2 * c,
-                             d_energy, gst);
-        // the next chunk reuses the arena, the partials and the metadata
+        ch.bps = std::max<uint64_t>(1, std::min<uint64_t>(64, S / 4096));
+        arena_need = std::max(arena_need, ch.c * S * B);
+        part_need = std::max<size_t>(part_need, (size_t)(ch.c * ch.bps));
+        chunks.push_back(ch);
       }
     }
-    hipError_t ge = gst ? hipStreamSynchronize(gst) : hipSuccess;
-    gclean();
-    if (ge != hipSuccess) {
-      cleanup();
-      return fail(QSV_E_DEVICE, std::string("batch: grouped passes failed: ") + hipGetErrorString(ge));
+    if (!chunks.empty()) {
+      hipStream_t gst = nullptr;
+      if ((!g_batch.gst && hipStreamCreateWithFlags(&g_batch.gst, hipStreamNonBlocking) != hipSuccess) ||
+          hipStreamWaitEvent(gst = g_batch.gst, uploaded, 0) != hipSuccess) {
+        cleanup();
+        return fail(QSV_E_DEVICE, "batch: group stream setup failed");
+      }
+      // the group stream is idle here (the previous call ended with a wait on
+      // it): growing and rewriting its buffers is safe
+      // the programs are gathered into pinned staging memory on the host
+      // threads, then copied up in one transfer
+      if ((rc = grow(g_batch.bytes_h, n_bytes))) {
+        cleanup();
+        return rc;
+      }
+      char* const staged = static_cast<char*>(g_batch.bytes_h.p);
+      parallel_over(copies.size(), [&](uint64_t k) {
+        const std::vector<char>& b = copies[k].first->bytes;
+        std::memcpy(staged + copies[k].second, b.data(), b.size());
+      });
+      if ((rc = grow_group_arena(arena_need)) || (rc = grow(g_batch.part, part_need * sizeof(double))) ||
+          (rc = upload(g_batch.bytes, staged, n_bytes, gst)) ||
+          (rc = upload(g_batch.offs, table.data(), table.size(), gst)) ||
+          (rc = upload(g_batch.meta, meta.data(), meta.size(), gst))) {
+        hipStreamSynchronize(gst);
+        cleanup();
+        return rc;
+      }
+      GroupLaunch gl{};
+      gl.prec = prec;
+      gl.arena = g_group_arena;
+      gl.progs = static_cast<const char*>(g_batch.bytes.p);
+      gl.table = static_cast<const int64_t*>(g_batch.offs.p);
+      gl.meta = static_cast<const uint64_t*>(g_batch.meta.p);
+      gl.part = static_cast<double*>(g_batch.part.p);
+      gl.tz = d_tz;
+      gl.tc = d_tc;
+      gl.energy = d_energy;
+      rc = run_group_launches(gl, chunks, gst);
+      hipError_t ge = hipStreamSynchronize(gst);
+      if (rc) {
+        cleanup();
+        return rc;
+      }
+      if (ge != hipSuccess) {
+        cleanup();
+        return fail(QSV_E_DEVICE, std::string("batch: grouped passes failed: ") + hipGetErrorString(ge));
+      }
     }
   }
   // programs that need a gate run alone: fused passes on pooled states,

@@ -208,6 +208,70 @@ void launch_batch_sample(int prec, const void* arena, const void* jobs, unsigned
                        (const SampleJob*)jobs, outcomes, norms);
 }
 
+constexpr uint32_t kZTerms = 256;   // Z strings staged in LDS per round
+constexpr int kZAmps = 4;           // amplitudes per thread per sweep over the staged strings
+
+// This thread's share of sum_j |a_j|^2 * sum_t c_t (-1)^popcount(j & z_t) over
+// the amplitudes j = first, first + stride, ... < S. The strings are read from
+// LDS (staged 256 at a time by the whole workgroup, so every thread of the
+// block must call this with the same t0, t1), each string against four
+// amplitudes per read.
+template <typename R>
+__device__ __forceinline__ double zstring_share(const typename BV<R>::T* __restrict__ psi, uint64_t S, uint64_t first,
+                                                uint64_t stride, const uint64_t* __restrict__ tz,
+                                                const double* __restrict__ tc, uint64_t t0, uint64_t t1)
+{
+  __shared__ uint64_t sz[kZTerms];
+  __shared__ double sc[kZTerms];
+  double acc = 0.0;
+  for (uint64_t c0 = t0; c0 < t1; c0 += kZTerms) {
+    const uint32_t nc = (uint32_t)(t1 - c0 < kZTerms ? t1 - c0 : kZTerms);
+    __syncthreads();   // the previous round's strings are no longer read
+    for (uint32_t k = threadIdx.x; k < nc; k += blockDim.x) {
+      sz[k] = tz[c0 + k];
+      sc[k] = tc[c0 + k];
+    }
+    __syncthreads();
+    for (uint64_t j = first; j < S; j += kZAmps * stride) {
+      uint64_t jj[kZAmps];
+      double p[kZAmps], h[kZAmps];
+#pragma unroll
+      for (int a = 0; a < kZAmps; ++a) {
+        jj[a] = j + (uint64_t)a * stride;
+        p[a] = 0.0;
+        h[a] = 0.0;
+        if (jj[a] < S) {
+          const auto v = psi[jj[a]];
+          p[a] = (double)v.x * v.x + (double)v.y * v.y;
+        }
+      }
+      if (S <= (1ull << 32)) {
+        // indices fit 32 bits: one AND and one popcount per string; the
+        // parity flips the sign bit of c
+#pragma unroll 4
+        for (uint32_t k = 0; k < nc; ++k) {
+          const uint32_t z = (uint32_t)sz[k];
+          const double c = sc[k];
+          const int chi = __double2hiint(c), clo = __double2loint(c);
+#pragma unroll
+          for (int a = 0; a < kZAmps; ++a)
+            h[a] += __hiloint2double(chi ^ (int)(__popc((uint32_t)jj[a] & z) << 31), clo);
+        }
+      } else {
+        for (uint32_t k = 0; k < nc; ++k) {
+          const uint64_t z = sz[k];
+          const double c = sc[k];
+#pragma unroll
+          for (int a = 0; a < kZAmps; ++a) h[a] += (__popcll(jj[a] & z) & 1) ? -c : c;
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < kZAmps; ++a) acc += p[a] * h[a];
+    }
+  }
+  return acc;
+}
+
 // <H> of a Z-string Hamiltonian on a state in global memory: per-block FP64
 // partials, then a fixed-order final sum into out[0].
 template <typename R>
@@ -216,15 +280,8 @@ __global__ __launch_bounds__(256) void k_zexpect(const typename BV<R>::T* __rest
                                                  uint64_t nterms, double* __restrict__ partials)
 {
   __shared__ double red[4];
-  double acc = 0.0;
   const uint64_t stride = (uint64_t)gridDim.x * 256;
-  for (uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x; j < S; j += stride) {
-    const auto a = psi[j];
-    const double p = (double)a.x * a.x + (double)a.y * a.y;
-    double h = 0.0;
-    for (uint64_t t = 0; t < nterms; ++t) h += (__popcll(j & tz[t]) & 1) ? -tc[t] : tc[t];
-    acc += p * h;
-  }
+  double acc = zstring_share<R>(psi, S, (uint64_t)blockIdx.x * 256 + threadIdx.x, stride, tz, tc, 0, nterms);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
@@ -258,15 +315,8 @@ __global__ __launch_bounds__(256) void k_zexpect_group(const typename BV<R>::T* 
   const uint64_t y = blockIdx.y;
   const auto* psi = base + y * S;
   const uint64_t t0 = trange[2 * y], t1 = trange[2 * y + 1];
-  double acc = 0.0;
   const uint64_t stride = (uint64_t)gridDim.x * 256;
-  for (uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x; j < S; j += stride) {
-    const auto a = psi[j];
-    const double p = (double)a.x * a.x + (double)a.y * a.y;
-    double h = 0.0;
-    for (uint64_t t = t0; t < t1; ++t) h += (__popcll(j & tz[t]) & 1) ? -tc[t] : tc[t];
-    acc += p * h;
-  }
+  double acc = zstring_share<R>(psi, S, (uint64_t)blockIdx.x * 256 + threadIdx.x, stride, tz, tc, t0, t1);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;

@@ -136,6 +136,9 @@ struct FStage {
 // a one-workgroup kernel turns the sums into scale = sqrt(target / norm^2),
 // and the next pass multiplies its loads by it (FP_RENORM_APPLY).
 enum { FP_RENORM_APPLY = 1, FP_RENORM_IN = 2, FP_RENORM_OUT = 4 };
+// A pass program flag beside them: the pass starts from |0..0> instead of
+// loading the state (the first pass round of a grouped batch).
+enum { FP_INIT_ZERO = 8 };
 
 struct TileNorm {
   double* part_in;    // per-tile input norm^2 (first pass of a run)

@@ -405,17 +405,26 @@ __global__ __launch_bounds__(kThreads, sizeof(A) == 4 ? 5 : sizeof(R) == 4 ? 4 :
     local_pending = false;
   };
 
+  const int rn = h.renorm;
   {
     uint64_t ga[kSlots];
     gaddr(lays[cur], ga);
-    each_slot([&](auto kk) __attribute__((always_inline)) {
-      constexpr int k = decltype(kk)::value;
-      const T v = __builtin_nontemporal_load(psi + ga[k]);
-      xr[k] = v.x;
-      xi[k] = v.y;
-    });
+    if (rn & FP_INIT_ZERO) {
+      // |0..0>: amplitude 1 at global index 0, nothing read
+      each_slot([&](auto kk) __attribute__((always_inline)) {
+        constexpr int k = decltype(kk)::value;
+        xr[k] = ga[k] == 0 ? (R)1 : (R)0;
+        xi[k] = (R)0;
+      });
+    } else {
+      each_slot([&](auto kk) __attribute__((always_inline)) {
+        constexpr int k = decltype(kk)::value;
+        const T v = __builtin_nontemporal_load(psi + ga[k]);
+        xr[k] = v.x;
+        xi[k] = v.y;
+      });
+    }
   }
-  const int rn = h.renorm;
   if (rn & FP_RENORM_APPLY) {
     const A sc = (A)nb.ctl[1];
     each_slot([&](auto kk) __attribute__((always_inline)) {

@@ -118,3 +118,50 @@ def test_batch_rejects_decreasing_offsets(eng):
             _abi.check(eng.lib().qsv_batch_expect_z(ctypes.byref(desc), out.ctypes.data))
     e = eng.batch_expect_z([(2, ops, [(1.0, "ZI"), (1.0, "IZ")])], "c128")
     assert abs(e[0]) < 1e-12
+
+
+def _graph_case(prec, shift=0.0):
+    """Grouped instances (15-17 qubits, one with no gates so that the arena is
+    filled) plus LDS-resident ones; `shift` moves every angle but keeps the
+    launch structure."""
+    from test_gpu_parity import _random_program
+    rng = np.random.default_rng(4242)
+    insts = []
+    for n, g in ((15, 120), (15, 0), (16, 90), (17, 60), (10, 40)):
+        ops = _random_program(rng, n, g) if g else qc.ops_from_tuples([])
+        ops = np.array(ops, copy=True)
+        if len(ops):
+            ops["angle"] += shift
+        terms = [(float(rng.normal()), "".join(rng.choice(["I", "Z"], size=n))) for _ in range(4)]
+        insts.append((n, ops, terms))
+    return insts
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_group_graph_replays_match_eager(eng, oracle, prec, monkeypatch):
+    """The grouped launches of a batch replayed as one captured graph: three
+    replays with other device work and host waits between them, then a replay
+    with new angles, each equal to the kernels issued one by one."""
+    insts = _graph_case(prec)
+    moved = _graph_case(prec, shift=0.37)
+    monkeypatch.setenv("QSV_BATCH_GRAPH", "0")
+    eager = eng.batch_expect_z(insts, prec)
+    eager_moved = eng.batch_expect_z(moved, prec)
+    monkeypatch.setenv("QSV_BATCH_GRAPH", "1")
+    other = [(16, qc.ops_from_tuples([(qc.OP["H"], q, 0, 0, 0, 0.0) for q in range(16)]), [(1.0, "Z" * 16)])]
+    for rep in range(3):
+        print(f"graph {prec} replay {rep}", file=sys.stderr, flush=True)
+        got = eng.batch_expect_z(insts, prec)
+        assert np.array_equal(got, eager), (rep, np.max(np.abs(got - eager)))
+        # device work of another shape between the replays
+        eng.batch_expect_z(other, prec)
+        with eng.DeviceState(12, prec) as s:
+            s.apply(qc.ops_from_tuples([(qc.OP["H"], 0, 0, 0, 0, 0.0)]))
+            s.probabilities()
+    got = eng.batch_expect_z(moved, prec)
+    assert np.array_equal(got, eager_moved), np.max(np.abs(got - eager_moved))
+    tol = 1e-10 if prec == "c128" else 1e-5
+    for (n, ops, terms), g in zip(moved, got):
+        psi = oracle.run(n, ops)
+        exp = sum(c * oracle.pauli_expectation(psi, n, s) for c, s in terms)
+        assert abs(g - exp) < tol * max(1.0, sum(abs(c) for c, _ in terms)), (n, g, exp)

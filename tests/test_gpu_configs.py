@@ -200,3 +200,36 @@ def test_c5_serving_large_requests(eng, oracle):
                 s.apply(ops, QSV_FLAG_FUSE)
                 ref = s.sample(shots, seed)
             assert np.mean(outs == ref) > 0.99, n
+
+
+def test_c3_4096_qaoa_batch_vs_oracle_stratified(eng, oracle):
+    """The batched QAOA workload at its full size: 4096 MaxCut instances
+    (n in {8..20} even, p in 1..4, gamma/beta uniform) in one complex64
+    batch call, as the benchmark runs it; every (n, p) cell is checked on
+    two instances against the FP64 oracle (sum_k c_k <P_k> in term order),
+    and every energy lies in [min cut term sum, max] bounds of the
+    Hamiltonian (|<H>| <= sum |c_k|)."""
+    insts = [wl.qaoa_instance(2026 * 100003 + k) for k in range(4096)]
+    batch = [(i["n"], wl.qaoa_ops(i), wl.qaoa_terms(i)) for i in insts]
+    note("C3: 4096-instance complex64 batch")
+    e64 = np.asarray(eng.PackedBatch(batch, "c64").run())
+    assert e64.shape == (4096,) and np.all(np.isfinite(e64))
+    bound = np.array([sum(abs(t["coefficient"]) for t in i["hamiltonian"]) for i in insts])
+    assert np.all(np.abs(e64) <= bound + 1e-6)
+    cells, picked = {}, []
+    for k, inst in enumerate(insts):
+        key = (inst["n"], inst["p"])
+        if cells.get(key, 0) < 2:
+            cells[key] = cells.get(key, 0) + 1
+            picked.append(k)
+    assert len(cells) == 28
+    worst = 0.0
+    for k in picked:
+        inst = insts[k]
+        psi = oracle.run(inst["n"], wl.qaoa_ops(inst))
+        ref = 0.0
+        for t in inst["hamiltonian"]:
+            ref += t["coefficient"] * oracle.pauli_expectation(psi, inst["n"], t["pauli-string"])
+        worst = max(worst, abs(ref - e64[k]))
+    note(f"C3: {len(picked)} instances over {len(cells)} cells, max |dE| vs FP64 oracle {worst:.2e}")
+    assert worst < 1e-5
