@@ -221,6 +221,16 @@ __device__ __forceinline__ void lds_sync(bool wave_local)
   }
 }
 
+// A wave-uniform value pinned to scalar registers: a per-lane choice between
+// two uniform loads would otherwise become a per-lane (vector) load.
+__device__ __forceinline__ double uniform(double v)
+{
+  const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+  const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float uniform(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
+
 // Coefficient i of an op in the arithmetic type.
 template <typename A> __device__ __forceinline__ A coef(const POp& op, int i);
 template <> __device__ __forceinline__ double coef<double>(const POp& op, int i) { return op.m[i]; }
@@ -254,7 +264,11 @@ __global__ __launch_bounds__(kThreads, sizeof(A) == 4 ? 5 : sizeof(R) == 4 ? 4 :
     psi += (uint64_t)blockIdx.y * S;
     prog = reinterpret_cast<const PProg*>(reinterpret_cast<const char*>(prog) + off);
   }
-  const uint32_t tile = blockIdx.x;
+  // The dispatcher hands consecutive workgroups to the 8 XCDs in turn: map
+  // them so that each XCD streams one contiguous eighth of the tiles (its own
+  // L2 and translation caches then see neighbouring addresses)
+  const uint32_t g = gridDim.x, b = blockIdx.x;
+  const uint32_t tile = (g & 7u) ? b : (b & 7u) * (g >> 3) + (b >> 3);
   const uint32_t t = threadIdx.x;
   const PHead& h = prog->h;
   const PLayout* lays = reinterpret_cast<const PLayout*>(reinterpret_cast<const char*>(prog) + h.lay_off);
@@ -474,8 +488,10 @@ __global__ __launch_bounds__(kThreads, sizeof(A) == 4 ? 5 : sizeof(R) == 4 ? 4 :
       // diagonal on a thread bit (aux = its thread-id bit): one factor per lane
       const bool hi = (t & op.aux) != 0;
       const bool on = act && (hi || op.only1 == 0);
-      const A dr = on ? (hi ? coef<A>(op, 6) : coef<A>(op, 0)) : (A)1;
-      const A di = on ? (hi ? coef<A>(op, 7) : coef<A>(op, 1)) : (A)0;
+      const A d0r = uniform(coef<A>(op, 0)), d0i = uniform(coef<A>(op, 1));
+      const A d1r = uniform(coef<A>(op, 6)), d1i = uniform(coef<A>(op, 7));
+      const A dr = on ? (hi ? d1r : d0r) : (A)1;
+      const A di = on ? (hi ? d1i : d0i) : (A)0;
       if (kc == 0) {
         each_slot([&](auto kk) __attribute__((always_inline)) {
           constexpr int k = decltype(kk)::value;
