@@ -885,7 +885,7 @@ int fuse_walk(const qsv_state* s, const std::vector<BGate>& gs, bool f32, Emit&&
   std::vector<uint64_t> gbits(N);
   for (size_t i = 0; i < N; ++i) gbits[i] = gate_bits(gs[i]);
   std::vector<size_t> chosen, win;
-  std::vector<uint64_t> cands;
+  std::vector<uint64_t> cands, wbits;   // wbits: the window's gate bits, contiguous
   constexpr size_t kWindow = 1024, kCandidates = 64;
   // gates a pass with tile bits `set` could take, in program order, from the
   // window: a gate whose bits are all in the set and that does not depend on a
@@ -893,8 +893,7 @@ int fuse_walk(const qsv_state* s, const std::vector<BGate>& gs, bool f32, Emit&&
   auto absorbed = [&](uint64_t set) -> int {
     uint64_t blocked = 0;
     int cnt = 0;
-    for (size_t i : win) {
-      const uint64_t gb = gbits[i];
+    for (const uint64_t gb : wbits) {
       if (!(gb & blocked) && !(gb & ~set)) {
         if (++cnt >= kMaxPassGates) break;
       } else {
@@ -923,10 +922,12 @@ int fuse_walk(const qsv_state* s, const std::vector<BGate>& gs, bool f32, Emit&&
     // (this pairs the partners of two-qubit gates instead of taking bits in
     // program order).
     win.clear();
+    wbits.clear();
     for (size_t i = first; i < N && win.size() < kWindow; ++i) {
       if (done[i]) continue;
       if (runs_alone(gs[i].kind)) break;
       win.push_back(i);
+      wbits.push_back(gbits[i]);
     }
     uint64_t set = lowmask | gbits[first];
     while (__builtin_popcountll(set) < TB) {
