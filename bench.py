@@ -518,7 +518,9 @@ def measure_layers(engine, n: int, prec: str, ops, args, tuned=None) -> dict:
     if not args.no_fused:
         from qclojure_amd._abi import QSV_FLAG_FUSE
         if tuned is None and "QSV_TILE_LOWBITS" not in os.environ:
-            tuned = engine.tune_fused(prec)
+            # tuned on a scratch state close to the workload size (the tuner takes up to 28 qubits)
+            tuned = engine.tune_fused(prec, num_qubits=max(16, min(n, 28)))
+            tuned["qubits"] = max(16, min(n, 28))
             log(f"bench: tuned tile low bits {tuned}")
         # the fused path takes the timed layers as one program, as a circuit is
         # submitted, so passes may span layer boundaries
@@ -544,7 +546,8 @@ def measure_layers(engine, n: int, prec: str, ops, args, tuned=None) -> dict:
                  if blk["launches"] else None,
                  "effective_GB_s": round(total_bytes / args.steps / (f_elapsed / args.steps) / 1e9, 1),
                  "tile_low_bits": tuned["low_bits"] if tuned else int(os.environ["QSV_TILE_LOWBITS"]),
-                 "tuner_ms_per_26q_layer": {str(k): round(v, 3) for k, v in tuned["ms_per_layer"].items()} if tuned else None,
+                 "tuner_qubits": tuned.get("qubits") if tuned else None,
+                 "tuner_ms_per_layer": {str(k): round(v, 3) for k, v in tuned["ms_per_layer"].items()} if tuned else None,
                  "note": "effective_GB_s = per-gate algorithmic bytes of the layer / fused layer time"}
         log(f"bench: {prec} fused {fused}")
     dev.destroy()

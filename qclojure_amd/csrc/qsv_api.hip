@@ -1907,9 +1907,8 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
     const int n = d->num_qubits[i];
     if (n < 1 || n > 34) return fail(QSV_E_INVALID, "instance " + std::to_string(i) + ": num_qubits out of range");
   }
-  // Decode every instance, convert its Z terms, and fuse the programs of the
-  // instances too large for LDS into grouped passes; then write the small
-  // instances' gate records into pinned staging memory. Both phases are
+  // Decode every instance, convert its Z terms and fold its gates; then write
+  // the small instances' gate records into pinned staging memory. Both phases are
   // independent per instance and run on a pool of host threads (at most 16);
   // errors are reported for the first failing instance in order, decode errors
   // before term errors, as a sequential pass would.
@@ -1919,7 +1918,6 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
     std::vector<int64_t> offs;
     bool ok = false;
   };
-  std::vector<Plan> plans(N);
   std::vector<int> dec_rc(N, 0);
   std::vector<std::string> dec_err(N);
   std::vector<char> term_bad(N, 0);
@@ -1964,7 +1962,6 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
       tc[k - T0] = d->term_coeff[k];
     }
     if (!dec_rc[i]) decoded[i] = fold_gates(decoded[i]);
-    if (!dec_rc[i] && n > lds_n) plans[i].ok = plan_group_passes(n, prec, decoded[i], plans[i].bytes, plans[i].offs);
   });
   for (uint64_t i = 0; i < N; ++i)
     if (dec_rc[i]) return fail(dec_rc[i], dec_err[i]);
@@ -1990,7 +1987,6 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
       *out++ = r;
     }
   });
-  std::vector<Plan> big_plans;
   for (uint64_t i = 0; i < N; ++i) {
     const int n = d->num_qubits[i];
     BInst& in = insts[i];
@@ -2007,7 +2003,6 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
       in.g0 = in.g1 = 0;
       big_idx.push_back(i);
       big_gates.push_back(std::move(decoded[i]));
-      big_plans.push_back(std::move(plans[i]));
     }
   }
   // largest circuits first, so the long workgroups start early
@@ -2054,7 +2049,13 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
   // circuits too large for LDS, grouped by qubit count: all instances of a
   // group sit back to back in one arena and advance one pass round per launch
   // (k_pass group mode, FP64 arithmetic); a program that needs a gate run
-  // alone takes a pooled state of its own instead
+  // alone takes a pooled state of its own instead. Their passes are planned
+  // here, on the host threads, while the LDS batch runs on the device.
+  std::vector<Plan> big_plans(big_idx.size());
+  parallel_over(big_idx.size(), [&](uint64_t k) {
+    Plan& pl = big_plans[k];
+    pl.ok = plan_group_passes(insts[big_idx[k]].n, prec, big_gates[k], pl.bytes, pl.offs);
+  });
   std::vector<qsv_state*> used;
   std::vector<size_t> singles;
   {
