@@ -518,9 +518,9 @@ def measure_layers(engine, n: int, prec: str, ops, args, tuned=None) -> dict:
     if not args.no_fused:
         from qclojure_amd._abi import QSV_FLAG_FUSE
         if tuned is None and "QSV_TILE_LOWBITS" not in os.environ:
-            # tuned on a scratch state close to the workload size (the tuner takes up to 28 qubits)
-            tuned = engine.tune_fused(prec, num_qubits=max(16, min(n, 28)))
-            tuned["qubits"] = max(16, min(n, 28))
+            # tuned on a scratch state of the workload's size (the tuner takes 16..30 qubits)
+            tuned = engine.tune_fused(prec, num_qubits=max(16, min(n, 30)))
+            tuned["qubits"] = max(16, min(n, 30))
             log(f"bench: tuned tile low bits {tuned}")
         # the fused path takes the timed layers as one program, as a circuit is
         # submitted, so passes may span layer boundaries
@@ -608,7 +608,7 @@ def main():
     other = "c64" if args.prec == "c128" else "c128"
     other_run = None
     if not args.no_other_precision:
-        other_run = measure_layers(engine, n, other, ops, args, tuned=main_run.get("tuned"))
+        other_run = measure_layers(engine, n, other, ops, args)
         other_run.pop("tuned", None)
     elapsed, stats, total_bytes = main_run["elapsed"], main_run["stats"], main_run["total_bytes"]
     value = total_bytes / elapsed / 1e9

@@ -3065,7 +3065,7 @@ int qsv_tune_fused(int precision, int num_qubits, qsv_tune_result* out)
   if (!out) return fail(QSV_E_INVALID, "null output");
   if (precision != QSV_C128 && precision != QSV_C64) return fail(QSV_E_INVALID, "unknown precision");
   const int n = num_qubits ? num_qubits : 26;
-  if (n < 16 || n > 28) return fail(QSV_E_INVALID, "tuning state must have 16..28 qubits");
+  if (n < 16 || n > 30) return fail(QSV_E_INVALID, "tuning state must have 16..30 qubits");
   std::memset(out, 0, sizeof(*out));
   qsv_state* s = nullptr;
   int rc = qsv_state_create(n, precision, &s);

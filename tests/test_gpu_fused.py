@@ -136,7 +136,7 @@ def test_tuner_picks_a_candidate_and_results_hold(eng, prec):
     b = _run(eng, n, ops, prec, flags=QSV_FLAG_FUSE)
     tol = 1e-12 if prec == "c128" else 1e-5
     assert np.max(np.abs(a - b)) < tol
-    with pytest.raises(Exception, match="16..28"):
+    with pytest.raises(Exception, match="16..30"):
         eng.tune_fused(prec, 40)
 
 
