@@ -378,6 +378,13 @@ int qsv_la_solve(const qsv_mat* A, const qsv_mat* B, qsv_mat* X);
  * order of V's columns; the caller sorts) and eigenvectors as the columns of
  * V, by parallel cyclic Jacobi; sweeps (may be NULL) reports the sweeps used. */
 int qsv_la_eigh(const qsv_mat* A, double* eigenvalues, qsv_mat* V, int* sweeps);
+/* eigen-general (protocols.clj:375-387, fastmath/complex_linear_algebra.clj:792)
+ * for a square complex A with n <= 64: eigenvalues as n (re, im) pairs (2n
+ * doubles, in the order of V's columns; the caller sorts) and unit
+ * eigenvectors as the columns of V, by Householder reduction to Hessenberg
+ * form and the shifted QR algorithm in one workgroup. Larger n fails with
+ * QSV_E_INVALID, a QR iteration that does not converge with QSV_E_DEVICE. */
+int qsv_la_eig(const qsv_mat* A, double* eigenvalues, qsv_mat* V);
 /* matrix-vector-product of a Kronecker product kept as its factors: x <-
  * (F_0 kron ... kron F_{n-1}) x, each factor a row-major 2x2 (re, im) or NULL
  * for the identity, factor k acting on reference qubit k. The shape the

@@ -182,6 +182,7 @@ SIGNATURES = {
     "qsv_la_deviation": (_i32, [_i32, _vp, _pd]),
     "qsv_la_solve": (_i32, [_vp, _vp, _vp]),
     "qsv_la_eigh": (_i32, [_vp, _pd, _vp, ctypes.POINTER(_i32)]),
+    "qsv_la_eig": (_i32, [_vp, _pd, _vp]),
     "qsv_la_kron_apply": (_i32, [ctypes.POINTER(_pd), _i32, _vp]),
     "qsv_la_cholesky": (_i32, [_vp, _dbl, _vp]),
     "qsv_la_lu": (_i32, [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_int64)]),

@@ -710,6 +710,205 @@ __global__ __launch_bounds__(256) void k_jacobi_small(const double2* __restrict_
   if (tid == 0) *sweeps_out = sweeps;
 }
 
+// eigen-general of a square complex matrix (n <= kJacSmall), one wavefront:
+// Householder reduction to upper Hessenberg form H = Q^H A Q, then the
+// shifted QR algorithm (implicit single shift, Wilkinson's shift from the
+// trailing 2x2, an exceptional shift every tenth iteration, deflation where
+// |h(l, l-1)| <= eps (|h(l-1, l-1)| + |h(l, l)|)) to the Schur form T = Z^H A Z,
+// then eigenvectors by back-substitution in T (column k of Y solves
+// (T - t_kk I) y = 0 with y_k = 1, y_j = 0 below) and X = Z Y, columns of unit
+// length. Lane i owns row or column i of each update. w: the eigenvalues
+// t_kk as (re, im) pairs; Y: n x n scratch; fail[0] = 1 when QR did not
+// converge within 30 iterations per eigenvalue.
+__device__ inline double cabs1(double2 a) { return hypot(a.x, a.y); }
+__device__ inline double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ inline double2 csqrt1(double2 a)
+{
+  const double r = cabs1(a);
+  if (r == 0.0) return make_double2(0.0, 0.0);
+  const double re = sqrt(0.5 * (r + fabs(a.x)));
+  const double im = 0.5 * a.y / re;
+  return a.x >= 0.0 ? make_double2(re, im) : make_double2(fabs(im), a.y >= 0.0 ? re : -re);
+}
+
+__global__ __launch_bounds__(64) void k_eig_general(const double2* __restrict__ A, double2* __restrict__ Xout,
+                                                    double2* __restrict__ Y, double* __restrict__ w, int n,
+                                                    int* __restrict__ fail)
+{
+  __shared__ double2 sH[kJacSmall][kJacPad];
+  __shared__ double2 sZ[kJacSmall][kJacPad];
+  __shared__ double2 sv[kJacSmall];
+  __shared__ double2 sw[kJacSmall];
+  const int l = threadIdx.x;
+  for (int e = l; e < n * n; e += 64) {
+    const int i = e / n, j = e - i * n;
+    sH[i][j] = A[e];
+    sZ[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+  }
+  __syncthreads();
+  // ---- Householder reduction: column k below the subdiagonal -> 0
+  for (int k = 0; k + 2 < n; ++k) {
+    const int m = n - k - 1;   // length of x = H[k+1 .. n-1][k]
+    double nx = 0.0, tail = 0.0;
+    for (int i = 0; i < m; ++i) {
+      const double a2 = cabs2(sH[k + 1 + i][k]);
+      nx += a2;
+      if (i) tail += a2;
+    }
+    if (tail == 0.0) continue;   // already Hessenberg in this column (all lanes agree)
+    nx = sqrt(nx);
+    const double2 x0 = sH[k + 1][k];
+    const double a0 = cabs1(x0);
+    const double2 ph = a0 > 0.0 ? cscale(x0, 1.0 / a0) : make_double2(1.0, 0.0);
+    const double2 alpha = cscale(ph, -nx);                // x -> alpha e_0
+    // v = x - alpha e_0, normalised
+    if (l < m) sv[l] = l == 0 ? csub(x0, alpha) : sH[k + 1 + l][k];
+    __syncthreads();
+    double nv = 0.0;
+    for (int i = 0; i < m; ++i) nv += cabs2(sv[i]);
+    const double inv = 1.0 / sqrt(nv);
+    __syncthreads();
+    if (l < m) sv[l] = cscale(sv[l], inv);
+    __syncthreads();
+    // H <- P H (P = I - 2 v v^H on rows k+1 ..): lane j takes column j
+    if (l < n && l >= k) {
+      double2 t = make_double2(0.0, 0.0);
+      for (int i = 0; i < m; ++i) t = cadd(t, cmul(cconj(sv[i]), sH[k + 1 + i][l]));
+      for (int i = 0; i < m; ++i) sH[k + 1 + i][l] = csub(sH[k + 1 + i][l], cscale(cmul(sv[i], t), 2.0));
+    }
+    __syncthreads();
+    // H <- H P and Z <- Z P: lane i takes row i
+    if (l < n) {
+      double2 t = make_double2(0.0, 0.0), u = make_double2(0.0, 0.0);
+      for (int j = 0; j < m; ++j) {
+        t = cadd(t, cmul(sH[l][k + 1 + j], sv[j]));
+        u = cadd(u, cmul(sZ[l][k + 1 + j], sv[j]));
+      }
+      for (int j = 0; j < m; ++j) {
+        sH[l][k + 1 + j] = csub(sH[l][k + 1 + j], cscale(cmul(t, cconj(sv[j])), 2.0));
+        sZ[l][k + 1 + j] = csub(sZ[l][k + 1 + j], cscale(cmul(u, cconj(sv[j])), 2.0));
+      }
+    }
+    __syncthreads();
+    if (l < m && l >= 1) sH[k + 1 + l][k] = make_double2(0.0, 0.0);
+    __syncthreads();
+  }
+  // ---- shifted QR on the Hessenberg matrix, deflating from the bottom
+  const double eps = 2.220446049250313e-16;
+  int hi = n - 1, iter = 0;
+  bool ok = true;
+  while (hi > 0) {
+    int lo = hi;
+    while (lo > 0) {
+      const double s = cabs1(sH[lo - 1][lo - 1]) + cabs1(sH[lo][lo]);
+      if (cabs1(sH[lo][lo - 1]) <= eps * (s > 0.0 ? s : 1.0)) break;
+      --lo;
+    }
+    __syncthreads();
+    if (lo > 0 && l == 0) sH[lo][lo - 1] = make_double2(0.0, 0.0);
+    __syncthreads();
+    if (lo == hi) {
+      --hi;
+      iter = 0;
+      continue;
+    }
+    if (++iter > 30) {
+      ok = false;
+      break;
+    }
+    // Wilkinson's shift: the eigenvalue of the trailing 2x2 nearer d
+    const double2 a = sH[hi - 1][hi - 1], b = sH[hi - 1][hi], c = sH[hi][hi - 1], d = sH[hi][hi];
+    double2 mu;
+    if (iter % 10 == 0) {
+      mu = cadd(d, make_double2(cabs1(c), 0.0));   // exceptional shift
+    } else {
+      const double2 hd = cscale(csub(a, d), 0.5);
+      const double2 disc = csqrt1(cadd(cmul(hd, hd), cmul(b, c)));
+      const double2 m1 = cadd(cscale(cadd(a, d), 0.5), disc), m2 = csub(cscale(cadd(a, d), 0.5), disc);
+      mu = cabs2(csub(m1, d)) <= cabs2(csub(m2, d)) ? m1 : m2;
+    }
+    // implicit single-shift QR sweep over rows lo .. hi
+    double2 x = csub(sH[lo][lo], mu), y = sH[lo + 1][lo];
+    for (int k = lo; k < hi; ++k) {
+      if (k > lo) {
+        x = sH[k][k - 1];
+        y = sH[k + 1][k - 1];
+      }
+      // G = [c s; -conj(s) c] with G (x, y) = (r, 0), c real
+      const double ax = cabs1(x), r = hypot(ax, cabs1(y));
+      double cr;
+      double2 sg;
+      if (r == 0.0) {
+        cr = 1.0;
+        sg = make_double2(0.0, 0.0);
+      } else if (ax == 0.0) {
+        cr = 0.0;
+        sg = cscale(cconj(y), 1.0 / cabs1(y));
+      } else {
+        cr = ax / r;
+        sg = cscale(cmul(cscale(x, 1.0 / ax), cconj(y)), 1.0 / r);
+      }
+      __syncthreads();
+      // rows k, k+1 of H (columns max(lo, k-1) .. n-1): lane j takes column j
+      const int j0 = k > lo ? k - 1 : lo;
+      if (l < n && l >= j0) {
+        const double2 p = sH[k][l], q = sH[k + 1][l];
+        sH[k][l] = cadd(cscale(p, cr), cmul(sg, q));
+        sH[k + 1][l] = csub(cscale(q, cr), cmul(cconj(sg), p));
+      }
+      __syncthreads();
+      // columns k, k+1 of H (rows 0 .. min(k+2, hi)) and of Z (all rows): G^H on the right
+      const int i1 = k + 2 < hi ? k + 2 : hi;
+      if (l < n) {
+        if (l <= i1) {
+          const double2 p = sH[l][k], q = sH[l][k + 1];
+          sH[l][k] = cadd(cscale(p, cr), cmul(q, cconj(sg)));
+          sH[l][k + 1] = csub(cscale(q, cr), cmul(p, sg));
+        }
+        const double2 p = sZ[l][k], q = sZ[l][k + 1];
+        sZ[l][k] = cadd(cscale(p, cr), cmul(q, cconj(sg)));
+        sZ[l][k + 1] = csub(cscale(q, cr), cmul(p, sg));
+      }
+      __syncthreads();
+    }
+    // the bulge of the chase is removed: keep the subdiagonal band exact
+    if (l < n && l >= lo + 2 && l <= hi) sH[l][l - 2] = make_double2(0.0, 0.0);
+    __syncthreads();
+  }
+  if (l == 0) fail[0] = ok ? 0 : 1;
+  if (l < n) sw[l] = sH[l][l];
+  __syncthreads();
+  // ---- eigenvectors of T (lane k: column k of Y, by back-substitution)
+  double tn = 0.0;
+  for (int i = 0; i < n; ++i) tn = fmax(tn, cabs1(sH[i][i]));
+  const double small = eps * (tn > 0.0 ? tn : 1.0);
+  if (l < n) {
+    const int k = l;
+    const double2 lam = sw[k];
+    for (int i = n - 1; i > k; --i) Y[(int64_t)i * n + k] = make_double2(0.0, 0.0);
+    Y[(int64_t)k * n + k] = make_double2(1.0, 0.0);
+    for (int i = k - 1; i >= 0; --i) {
+      double2 t = make_double2(0.0, 0.0);
+      for (int j = i + 1; j <= k; ++j) t = cadd(t, cmul(sH[i][j], Y[(int64_t)j * n + k]));
+      double2 den = csub(sH[i][i], lam);
+      if (cabs1(den) < small) den = make_double2(small, 0.0);
+      Y[(int64_t)i * n + k] = cdiv(make_double2(-t.x, -t.y), den);
+    }
+    // X = Z Y, column k, unit length
+    double nrm = 0.0;
+    for (int i = 0; i < n; ++i) {
+      double2 t = make_double2(0.0, 0.0);
+      for (int j = 0; j <= k; ++j) t = cadd(t, cmul(sZ[i][j], Y[(int64_t)j * n + k]));
+      Xout[(int64_t)i * n + k] = t;
+      nrm += cabs2(t);
+    }
+    const double inv = nrm > 0.0 ? 1.0 / sqrt(nrm) : 1.0;
+    for (int i = 0; i < n; ++i) Xout[(int64_t)i * n + k] = cscale(Xout[(int64_t)i * n + k], inv);
+    w[2 * k] = lam.x;
+    w[2 * k + 1] = lam.y;
+  }
+}
+
 __global__ void k_diag_real(const double2* __restrict__ M, int64_t n, double* __restrict__ w)
 {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -1237,6 +1436,40 @@ int qsv_la_solve(const qsv_mat* A, const qsv_mat* B, qsv_mat* X)
     return done(api_fail(QSV_E_INVALID, msg.c_str()));
   }
   return done(QSV_OK);
+}
+
+int qsv_la_eig(const qsv_mat* A, double* eigenvalues, qsv_mat* V)
+{
+  int rc = check_mat(A, "A");
+  if (rc || (rc = check_mat(V, "V"))) return rc;
+  if (!eigenvalues) return api_fail(QSV_E_INVALID, "null eigenvalue buffer");
+  const int64_t n = A->rows;
+  if (A->cols != n) return api_fail(QSV_E_INVALID, "Matrix must be square for eigendecomposition");
+  if (V->rows != n || V->cols != n) return api_fail(QSV_E_INVALID, "eigen-general: V shape differs");
+  if (n < 1 || n > kJacSmall) return api_fail(QSV_E_INVALID, "eigen-general: the :hip backend takes n <= 64");
+  if (V->data == A->data) return api_fail(QSV_E_INVALID, "eigen-general: V aliases A");
+  std::lock_guard<std::mutex> lk(g_la_mu);
+  double2* Y = nullptr;
+  double* w = nullptr;
+  if ((rc = alloc(&Y, (uint64_t)n * n))) return rc;
+  if (hipMalloc(&w, (size_t)2 * n * sizeof(double) + sizeof(int)) != hipSuccess) {
+    hipFree(Y);
+    return api_fail(QSV_E_NOMEM, "eigen-general scratch allocation failed");
+  }
+  int* dfail = reinterpret_cast<int*>(w + 2 * n);
+  int failed = 1;
+  hipLaunchKernelGGL(k_eig_general, dim3(1), dim3(64), 0, g_la_stream, (const double2*)A->data, V->data, Y, w, (int)n,
+                     dfail);
+  const bool ok = hipGetLastError() == hipSuccess &&
+                  hipMemcpyAsync(eigenvalues, w, (size_t)2 * n * sizeof(double), hipMemcpyDeviceToHost, g_la_stream) ==
+                      hipSuccess &&
+                  hipMemcpyAsync(&failed, dfail, sizeof(int), hipMemcpyDeviceToHost, g_la_stream) == hipSuccess &&
+                  hipStreamSynchronize(g_la_stream) == hipSuccess;
+  hipFree(Y);
+  hipFree(w);
+  if (!ok) return api_fail(QSV_E_DEVICE, "eigen-general failed on the device");
+  if (failed) return api_fail(QSV_E_DEVICE, "eigen-general: the QR iteration did not converge");
+  return QSV_OK;
 }
 
 int qsv_la_eigh(const qsv_mat* A, double* eigenvalues, qsv_mat* V, int* sweeps_out)

@@ -7,7 +7,7 @@ complex_linear_algebra.clj).
 libqsv call per operation, and on top of those device calls svd, the matrix
 functions (exp, log, sqrt) and
 MatrixAnalysis, and LU, QR and Cholesky as single-workgroup kernels
-(protocols.clj:357-520); eigen-general raises LinearAlgebraError; its matrices are opaque device handles
+(protocols.clj:357-520) and eigen-general (n <= 64); its matrices are opaque device handles
 (``DeviceMatrix``), which the protocol allows ("implementations may return
 opaque handles"), so chained operations stay in HBM and only ``to_numpy()``
 reads them back. ``kronecker-product`` results are kept as their factors
@@ -537,10 +537,30 @@ class HipBackend:
             U = self.add(U, DeviceMatrix.from_numpy(E))
         return {"U": U, "S": [complex(sv[j], 0.0) for j in order], "V†": self.conjugate_transpose(Vs)}
 
-    def eigen_general(self, A):
-        raise LinearAlgebraError("eigen-general is not provided by the :hip backend",
-                                 {"provided": ["eigen-hermitian", "svd", "lu-decomposition", "qr-decomposition",
-                                               "cholesky-decomposition"]})
+    def eigen_general(self, A) -> dict:
+        """eigen-general (protocols.clj:375-387; fastmath/complex_linear_algebra.clj:792-870):
+        a Hermitian A (within 1e-10) goes to eigen-hermitian, as the
+        reference's does; any other square A up to 64 x 64 is reduced to
+        Hessenberg form and iterated to its Schur form on the device
+        (qsv_la_eig). Returns the n eigenvalues in ascending (re, im) order
+        with unit eigenvector columns. (The reference instead takes the n
+        smallest of the 2n eigenvalues of its real embedding, which repeats or
+        conjugates eigenvalues of a non-Hermitian A; that output is not
+        reproduced.)"""
+        A = self._dense(A)
+        if A.rows != A.cols:
+            raise LinearAlgebraError("Matrix must be square for eigendecomposition", {"shape": A.shape})
+        if self.hermitian(A, 1e-10):
+            return self.eigen_hermitian(A)
+        n = A.rows
+        w = np.empty(2 * n, dtype=np.float64)
+        V = DeviceMatrix(n, n)
+        _call(_lib().qsv_la_eig, A.handle, w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), V.handle)
+        lam = w[0::2] + 1j * w[1::2]
+        order = sorted(range(n), key=lambda i: (lam[i].real, lam[i].imag))
+        vecs = V.to_numpy()
+        return {"eigenvalues": [complex(lam[i]) for i in order],
+                "eigenvectors": [vecs[:, i].copy() for i in order]}
 
     def lu_decomposition(self, A) -> dict:
         """lu-decomposition (:965-1046): {"L", "U", "P"} with P·A = L·U,

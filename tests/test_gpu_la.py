@@ -265,8 +265,8 @@ def test_reference_function_cases(hip):
         assert la.condition_number([[1.0, 1.0], [1.0, 1.001]]) > 100
         s = la.svd(np.diag([2.0, 3.0, 1.0]))["singular-values"]
         assert len(s) == 3 and s[0].real >= s[1].real >= s[2].real
-        with pytest.raises(la.LinearAlgebraError):
-            la.eigen_general(np.eye(2))
+        e = la.eigen_general(np.eye(2))              # Hermitian: eigen-hermitian, as the reference does
+        assert np.allclose(e["eigenvalues"], [1.0, 1.0], atol=1e-14)
         with pytest.raises(la.LinearAlgebraError):
             la.matrix_log(np.array([[1.0, 2.0], [0.5, 1.0]]))   # singular
 
@@ -382,3 +382,66 @@ def test_decomposition_edge_cases(hip):
     K = hip.kronecker_product(np.array([[2.0, 1.0], [1.0, 2.0]]), np.eye(2))
     L = A_(hip.cholesky_decomposition(K)["L"])
     assert np.allclose(L @ L.conj().T, np.kron([[2.0, 1.0], [1.0, 2.0]], np.eye(2)), atol=1e-14)
+
+
+def _by_re_im(vals):
+    return np.array(sorted(vals, key=lambda z: (z.real, z.imag)))
+
+
+def _match_err(got, ref):
+    """Largest distance between two eigenvalue lists matched greedily
+    (pairs with equal real parts may come in either order)."""
+    ref = list(ref)
+    worst = 0.0
+    for z in got:
+        k = int(np.argmin([abs(z - r) for r in ref]))
+        worst = max(worst, abs(z - ref.pop(k)))
+    return worst
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 16, 40, 64])
+def test_eigen_general(hip, n):
+    """eigen-general of non-Hermitian complex matrices: the n eigenvalues in
+    ascending (re, im) order against LAPACK (numpy.linalg.eigvals), and each
+    unit eigenvector satisfies A v = lambda v."""
+    note(f"la: eigen-general n={n}")
+    rng = np.random.default_rng(300 + n)
+    A = rand_c(rng, n, n)
+    e = hip.eigen_general(A)
+    lam = np.array(e["eigenvalues"])
+    V = np.array(e["eigenvectors"]).T
+    ref = _by_re_im(np.linalg.eigvals(A))
+    scale = max(1.0, np.linalg.norm(A, 2))
+    assert lam.shape == (n,) and V.shape == (n, n)
+    assert all((a.real, a.imag) <= (b.real, b.imag) for a, b in zip(lam, lam[1:]))
+    assert _match_err(lam, ref) < 1e-10 * scale
+    assert np.max(np.abs(np.linalg.norm(V, axis=0) - 1.0)) < 1e-12
+    assert np.max(np.abs(A @ V - V * lam)) < 1e-9 * scale
+
+
+def test_eigen_general_special_inputs(hip):
+    """Real matrices with complex-conjugate eigenvalues, a triangular input
+    (already in Schur form), a Hermitian input (handed to eigen-hermitian, as
+    the reference does), and the size limit."""
+    note("la: eigen-general special inputs")
+    R = np.array([[0.0, -1.0], [1.0, 0.0]], dtype=complex)          # rotation: -i, +i
+    e = hip.eigen_general(R)
+    assert _match_err(e["eigenvalues"], [-1j, 1j]) < 1e-14
+    T = np.array([[3.0, 2.0, 1j], [0.0, 1.0, 5.0], [0.0, 0.0, -2.0]], dtype=complex)
+    e = hip.eigen_general(T)
+    assert np.allclose(e["eigenvalues"], [-2.0, 1.0, 3.0], atol=1e-13)
+    V = np.array(e["eigenvectors"]).T
+    assert np.max(np.abs(T @ V - V * np.array(e["eigenvalues"]))) < 1e-12
+    rng = np.random.default_rng(7)
+    H = rand_c(rng, 5, 5)
+    H = H + H.conj().T
+    g, h = hip.eigen_general(H), hip.eigen_hermitian(H)
+    assert np.allclose(g["eigenvalues"], h["eigenvalues"], atol=1e-13)
+    # a real non-symmetric matrix: conjugate pairs, against LAPACK
+    M = rng.standard_normal((9, 9)).astype(complex)
+    e = hip.eigen_general(M)
+    assert _match_err(e["eigenvalues"], np.linalg.eigvals(M)) < 1e-10 * np.linalg.norm(M, 2)
+    with pytest.raises(la.LinearAlgebraError, match="n <= 64"):
+        hip.eigen_general(rand_c(rng, 65, 65))
+    with pytest.raises(la.LinearAlgebraError, match="square"):
+        hip.eigen_general(rand_c(rng, 3, 4))
