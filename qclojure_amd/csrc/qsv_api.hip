@@ -873,8 +873,14 @@ void end_run(std::vector<FPass>& passes)
 // pass, barrier(i) each gate that must run alone (a measurement, or a gate
 // wider than a tile); both return a status. With f32, every pass gets the
 // renormalisation flags of its position in its run of passes.
+// The passes fuse_walk chose for a program: per pass, its gates (indices into
+// the program) and its tile bit set. build_pass on the same gates and set
+// rebuilds the pass, whatever the gates' matrices.
+using PassShape = std::vector<std::pair<std::vector<uint32_t>, uint64_t>>;
+
 template <class Emit, class Barrier>
-int fuse_walk(const qsv_state* s, const std::vector<BGate>& gs, bool f32, Emit&& emit, Barrier&& barrier)
+int fuse_walk(const qsv_state* s, const std::vector<BGate>& gs, bool f32, Emit&& emit, Barrier&& barrier,
+              PassShape* record = nullptr)
 {
   const int TB = s->prec == QSV_C128 ? kTileBitsC128 : kTileBitsC64;
   const int L = tile_low_bits(s);
@@ -972,6 +978,7 @@ int fuse_walk(const qsv_state* s, const std::vector<BGate>& gs, bool f32, Emit&&
     FPass p;
     const size_t taken = build_pass(s, gs, chosen, set, p);
     for (size_t i = 0; i < taken; ++i) done[chosen[i]] = 1;
+    if (record) record->emplace_back(std::vector<uint32_t>(chosen.begin(), chosen.begin() + taken), set);
     p.renorm = f32 ? (FP_RENORM_OUT | (run_first ? FP_RENORM_IN : FP_RENORM_APPLY)) : 0;
     run_first = false;
     if ((rc = emit(p))) return rc;
@@ -1150,6 +1157,47 @@ int run_fused(qsv_state* s, const std::vector<BGate>& gs_in, uint32_t flags)
   return launch_passes(s, passes, f32);
 }
 
+// Pass shapes of grouped batch programs, by program structure: an optimiser
+// loop resubmits the same circuits with new angles, and the fuser's choices
+// depend only on which bits each gate touches (and the tile width), so a
+// structure seen before skips the fuser's search. Keyed by the exact
+// structure (hash plus full comparison); cleared when it grows past a bound.
+struct ShapeEntry {
+  std::vector<uint64_t> key;
+  PassShape shape;
+};
+std::mutex g_shape_mu;
+std::unordered_map<uint64_t, std::vector<ShapeEntry>> g_shapes;
+size_t g_shape_count = 0;
+constexpr size_t kMaxShapes = 16384;
+
+std::vector<uint64_t> structure_key(int n, int prec, int L, const std::vector<BGate>& gs)
+{
+  std::vector<uint64_t> k;
+  k.reserve(3 + 3 * gs.size());
+  k.push_back((uint64_t)n | ((uint64_t)prec << 8) | ((uint64_t)L << 16));
+  for (const BGate& g : gs) {
+    k.push_back((uint64_t)(uint32_t)g.kind | ((uint64_t)(uint32_t)(g.a + 1) << 16) |
+                ((uint64_t)(uint32_t)(g.b + 1) << 32) | ((uint64_t)g.cbits.size() << 48));
+    uint64_t c = 0;
+    for (int x : g.cbits) c = (c << 8) | (uint64_t)(x + 1);
+    k.push_back(c);
+    k.push_back(g.bmask);
+  }
+  return k;
+}
+
+uint64_t key_hash(const std::vector<uint64_t>& k)
+{
+  uint64_t h = 1469598103934665603ull;
+  for (uint64_t v : k) {
+    h ^= v;
+    h *= 1099511628211ull;
+    h ^= h >> 29;
+  }
+  return h;
+}
+
 // Passes of one instance of a grouped batch (FP64 arithmetic, no
 // renormalisation), appended to `bytes` truncated after the last gate; false
 // if the program needs a gate run alone.
@@ -1160,30 +1208,68 @@ bool plan_group_passes(int n, int prec, const std::vector<BGate>& gs, std::vecto
   view.n = n;
   view.prec = prec;
   view.S = 1ull << n;
-  bool ok = true;
   PProg prog;
+  auto append = [&](const FPass& p) {
+    // header and ops, then the layout table right after the last op; the
+    // instance's first pass writes |0..0> itself
+    const size_t ops_end = compile_pass(p, prog);
+    if (offsets.empty()) prog.h.renorm |= FP_INIT_ZERO;
+    const size_t lay_bytes = (size_t)prog.h.nlay * sizeof(PLayout);
+    const size_t lay_at = (ops_end + 15) & ~(size_t)15;
+    prog.h.lay_off = (int32_t)lay_at;
+    const size_t len = (lay_at + lay_bytes + 255) & ~(size_t)255;
+    const size_t off = bytes.size();
+    bytes.resize(off + len, 0);
+    std::memcpy(bytes.data() + off, &prog, ops_end);
+    std::memcpy(bytes.data() + off + lay_at, prog.lay, lay_bytes);
+    offsets.push_back((int64_t)off);
+  };
+  std::vector<uint64_t> key = structure_key(n, prec, tile_low_bits(&view), gs);
+  const uint64_t h = key_hash(key);
+  PassShape shape;
+  bool hit = false;
+  {
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    auto it = g_shapes.find(h);
+    if (it != g_shapes.end())
+      for (const ShapeEntry& e : it->second)
+        if (e.key == key) {
+          shape = e.shape;
+          hit = true;
+          break;
+        }
+  }
+  if (hit) {
+    std::vector<size_t> chosen;
+    for (const auto& ps : shape) {
+      chosen.assign(ps.first.begin(), ps.first.end());
+      FPass p;
+      if (build_pass(&view, gs, chosen, ps.second, p) != chosen.size()) return false;   // cannot happen: same structure
+      append(p);
+    }
+    return true;
+  }
+  bool ok = true;
   fuse_walk(
       &view, gs, false,
       [&](const FPass& p) {
-        // header and ops, then the layout table right after the last op; the
-        // instance's first pass writes |0..0> itself
-        const size_t ops_end = compile_pass(p, prog);
-        if (offsets.empty()) prog.h.renorm |= FP_INIT_ZERO;
-        const size_t lay_bytes = (size_t)prog.h.nlay * sizeof(PLayout);
-        const size_t lay_at = (ops_end + 15) & ~(size_t)15;
-        prog.h.lay_off = (int32_t)lay_at;
-        const size_t len = (lay_at + lay_bytes + 255) & ~(size_t)255;
-        const size_t off = bytes.size();
-        bytes.resize(off + len, 0);
-        std::memcpy(bytes.data() + off, &prog, ops_end);
-        std::memcpy(bytes.data() + off + lay_at, prog.lay, lay_bytes);
-        offsets.push_back((int64_t)off);
+        append(p);
         return QSV_OK;
       },
       [&](size_t) {
         ok = false;
         return QSV_E_UNSUPPORTED;
-      });
+      },
+      &shape);
+  if (ok) {
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    if (g_shape_count >= kMaxShapes) {
+      g_shapes.clear();
+      g_shape_count = 0;
+    }
+    g_shapes[h].push_back(ShapeEntry{std::move(key), std::move(shape)});
+    ++g_shape_count;
+  }
   return ok;
 }
 
@@ -1855,6 +1941,11 @@ void batch_cache_release()
   for (GroupGraph& gg : g_batch.graphs)
     if (gg.exec) hipGraphExecDestroy(gg.exec);
   g_batch.graphs.clear();
+  {
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    g_shapes.clear();
+    g_shape_count = 0;
+  }
   for (GrowBuf* b : {&g_batch.recs_h, &g_batch.bytes_h, &g_batch.recs, &g_batch.insts, &g_batch.tz, &g_batch.tc,
                      &g_batch.order, &g_batch.energy, &g_batch.bytes, &g_batch.offs, &g_batch.meta, &g_batch.part}) {
     if (b->p) (void)(b->pinned ? hipHostFree(b->p) : hipFree(b->p));
