@@ -592,7 +592,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the state-vector path "
-              "here runs on a single MI355X; sharding is described in DESIGN.md)", file=sys.stderr)
+              "here runs on a single MI355X; multi-GPU sharding is not built)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
