@@ -81,6 +81,9 @@ typedef struct qsv_op {
 #define QSV_FLAG_FP64_PASSES 0x4u /* complex64 tiled passes compute in FP64. By default
                                     they compute in FP32 and remove the resulting norm
                                     drift pass by pass (FP64 per-tile norm sums). */
+#define QSV_FLAG_VALU_PASSES 0x8u /* FP32 complex64 passes keep every gate on the vector
+                                    units. By default runs of register-bit gates in a
+                                    stage become dense 16x16 blocks on FP32 MFMA. */
 
 /* ---- library / device --------------------------------------------------- */
 typedef struct qsv_device_info {

@@ -29,6 +29,7 @@ QSV_C64 = 1
 QSV_FLAG_MSB_SWAP = 0x1
 QSV_FLAG_FUSE = 0x2
 QSV_FLAG_FP64_PASSES = 0x4
+QSV_FLAG_VALU_PASSES = 0x8
 
 _KIND_NAMES = [
     "X", "Y", "Z", "H", "S", "SDG", "T", "TDG", "RX", "RY", "RZ", "PHASE",

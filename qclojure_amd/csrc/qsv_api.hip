@@ -9,6 +9,7 @@
 #include <map>
 #include <atomic>
 #include <chrono>
+#include <complex>
 #include <cmath>
 #include <condition_variable>
 #include <cstdio>
@@ -592,7 +593,150 @@ void make_layout(uint32_t reg, const int32_t pos[16], PLayout& L)
   }
 }
 
-size_t compile_pass(const FPass& p, PProg& out)
+// A PO_BLOCK costs a wave 64 v_mfma_f32_16x16x4f32 (128 flops per amplitude,
+// 0.9 ms per 2^30 amplitudes at the FP32 matrix peak) plus an LDS round trip,
+// whatever it absorbs, so it pays only for long runs of expensive ops. Vector
+// instructions per wave of each register-bit op in an FP32 pass (dense real,
+// dense general and two-sided diagonal measured with one op kind per pass;
+// the exchange and the one-sided diagonal estimated), and the least total a
+// block must replace (QSV_PASS_BLOCKS; 0 keeps every op on the vector units):
+// on complex64 30-qubit passes, blocks of three or more ops ran 15-40 %
+// slower than the vector units, blocks of six or more 2 % faster on U3/CZ
+// layers.
+int reg_op_cost(const POp& op)
+{
+  const int c = op.code;
+  if (c < PO_PHASE) {
+    const int dm = c & 3;
+    return dm == PM_GENERAL ? 147 : dm == PM_EXCHANGE ? 40 : 83;
+  }
+  if (c < PO_PHASE_T) return ((c - PO_PHASE) & 4) ? 60 : 101;
+  return 101;   // PO_BLOCKADE
+}
+
+int pass_block_cost()
+{
+  const char* e = std::getenv("QSV_PASS_BLOCKS");   // read per pass: a test may lower it
+  return e ? std::max(0, std::atoi(e)) : 850;
+}
+
+// Slot bits an op of a stage reads or writes; `reg_only`: it has no
+// thread-bit part, so it is the same 16 x 16 map on every thread's slots.
+uint32_t op_slot_bits(const POp& op, bool* reg_only)
+{
+  const int c = op.code;
+  if (c < PO_PHASE) {
+    *reg_only = op.tmask == 0;
+    return (1u << (c >> 2)) | op.kc;
+  }
+  if (c < PO_PHASE_T) {
+    *reg_only = op.tmask == 0;
+    return (1u << ((c - PO_PHASE) & 3)) | op.kc;
+  }
+  *reg_only = c == PO_BLOCKADE && op.aux == 0;
+  return op.kc;   // PO_PHASE_T, PO_BLOCKADE
+}
+
+// v = (map of a register-bit op) v on one thread's 16 slots, as k_pass applies it.
+void apply_reg_op(const POp& op, std::complex<double> v[16])
+{
+  using C = std::complex<double>;
+  const int c = op.code;
+  const uint32_t kc = op.kc;
+  if (c < PO_PHASE) {
+    const int d = 1 << (c >> 2);
+    const C m00(op.m[0], op.m[1]), m01(op.m[2], op.m[3]), m10(op.m[4], op.m[5]), m11(op.m[6], op.m[7]);
+    for (int k = 0; k < 16; ++k) {
+      if ((k & d) || (k & kc) != kc) continue;
+      const C a = v[k], b = v[k | d];
+      v[k] = m00 * a + m01 * b;
+      v[k | d] = m10 * a + m11 * b;
+    }
+  } else if (c < PO_PHASE_T) {
+    const int d = 1 << ((c - PO_PHASE) & 3);
+    const bool only1 = ((c - PO_PHASE) & 4) != 0;
+    const C d0(op.m[0], op.m[1]), d1(op.m[6], op.m[7]);
+    for (int k = 0; k < 16; ++k) {
+      if ((k & kc) != kc) continue;
+      if (k & d) v[k] *= d1;
+      else if (!only1) v[k] *= d0;
+    }
+  } else {   // PO_BLOCKADE without a thread part
+    const C ph(op.m[0], op.m[1]);
+    for (int k = 0; k < 16; ++k)
+      if (__builtin_popcount((uint32_t)k & kc) == 1) v[k] *= ph;
+  }
+}
+
+// Fold the register-bit ops of one stage (ops [n0, n) of `out`) into dense
+// 16 x 16 blocks for the matrix cores. Scanning in order, an op without a
+// thread-bit part joins the open block if its slot bits miss those of every
+// op set aside since the block opened; an op with a thread-bit part is set
+// aside and runs after the block (ops on disjoint bits commute). A block
+// whose ops cost the vector units less than min_cost keeps its ops in
+// program order. Returns the new op count.
+int form_blocks(PProg& out, int n0, int n, int& nblk, int min_cost)
+{
+  const std::vector<POp> src(out.ops + n0, out.ops + n);
+  std::vector<int> blk, aside;
+  uint32_t aside_bits = 0;
+  int w = n0, cost = 0;
+  auto flush = [&]() {
+    if (cost >= min_cost && nblk < kMaxPassBlocks) {
+      std::complex<double> U[16][16];
+      for (int col = 0; col < 16; ++col) {
+        std::complex<double> v[16];
+        for (int k = 0; k < 16; ++k) v[k] = k == col ? 1.0 : 0.0;
+        for (int i : blk) apply_reg_op(src[i], v);
+        for (int k = 0; k < 16; ++k) U[k][col] = v[k];
+      }
+      float* tab = out.blk[nblk];
+      for (int kk = 0; kk < 4; ++kk)
+        for (int l = 0; l < 64; ++l) {
+          const std::complex<double> u = U[l & 15][4 * kk + (l >> 4)];
+          tab[2 * (kk * 64 + l)] = (float)u.real();
+          tab[2 * (kk * 64 + l) + 1] = (float)u.imag();
+        }
+      POp& b = out.ops[w++];
+      std::memset(&b, 0, sizeof(b));
+      b.code = PO_BLOCK;
+      b.aux = (uint32_t)(offsetof(PProg, blk) + (size_t)nblk * sizeof(out.blk[0]));
+      ++nblk;
+      for (int i : aside) out.ops[w++] = src[i];
+    } else {
+      std::vector<int> all(blk);
+      all.insert(all.end(), aside.begin(), aside.end());
+      std::sort(all.begin(), all.end());
+      for (int i : all) out.ops[w++] = src[i];
+    }
+    blk.clear();
+    aside.clear();
+    aside_bits = 0;
+    cost = 0;
+  };
+  for (int i = 0; i < (int)src.size(); ++i) {
+    const POp& op = src[i];
+    if (op.code == PO_SWAP || op.code == PO_RELAYOUT) {
+      flush();
+      out.ops[w++] = op;
+      continue;
+    }
+    bool reg_only = false;
+    const uint32_t sb = op_slot_bits(op, &reg_only);
+    if (reg_only) {
+      if (sb & aside_bits) flush();
+      blk.push_back(i);
+      cost += reg_op_cost(op);
+    } else {
+      aside.push_back(i);
+      aside_bits |= sb;
+    }
+  }
+  flush();
+  return w;
+}
+
+size_t compile_pass(const FPass& p, PProg& out, bool blocks)
 {
   constexpr uint32_t FULL = (1u << kPassTileBits) - 1u;
   std::memset(&out.h, 0, sizeof(out.h));
@@ -604,7 +748,8 @@ size_t compile_pass(const FPass& p, PProg& out)
   make_layout(p.load_mask, p.pos, out.lay[nl]);
   out.h.load_lay = nl++;
   int cur = out.h.load_lay;
-  int n = 0;
+  int n = 0, nblk = 0;
+  const int block_cost = blocks ? pass_block_cost() : 0;
   uint32_t reg = p.load_mask;
   auto emit = [&]() -> POp& {
     POp& op = out.ops[n++];
@@ -637,6 +782,7 @@ size_t compile_pass(const FPass& p, PProg& out)
         if (((local & nonreg) >> b) & 1u) m |= 1u << rank_in(nonreg, b);
       return m;
     };
+    const int n0 = n;
     for (int gi = st.g0; gi < st.g1; ++gi) {
       const FGate& g = p.gates[gi];
       POp& op = emit();
@@ -672,6 +818,7 @@ size_t compile_pass(const FPass& p, PProg& out)
         break;
       }
     }
+    if (block_cost > 0) n = form_blocks(out, n0, n, nblk, block_cost);
   }
   out.h.nops = n;
   if (reg == p.store_mask) {
@@ -686,7 +833,7 @@ size_t compile_pass(const FPass& p, PProg& out)
   return offsetof(PProg, ops) + (size_t)n * sizeof(POp);
 }
 
-int upload_passes(qsv_state* s, const std::vector<FPass>& passes, PProg** dev)
+int upload_passes(qsv_state* s, const std::vector<FPass>& passes, bool blocks, PProg** dev)
 {
   auto& slot = s->ring[s->ring_next];
   s->ring_next = (s->ring_next + 1) % qsv_state::kRing;
@@ -703,14 +850,15 @@ int upload_passes(qsv_state* s, const std::vector<FPass>& passes, PProg** dev)
     slot.cap = cap;
   }
   if (!slot.event) HIPCHK(hipEventCreateWithFlags(&slot.event, hipEventDisableTiming));
-  for (size_t i = 0; i < need; ++i) compile_pass(passes[i], slot.host[i]);
+  for (size_t i = 0; i < need; ++i) compile_pass(passes[i], slot.host[i], blocks);
   HIPCHK(hipMemcpyAsync(slot.dev, slot.host, need * sizeof(PProg), hipMemcpyHostToDevice, s->stream));
   *dev = slot.dev;
   s->ring_pending = &slot;
   return QSV_OK;
 }
 
-int launch_passes(qsv_state* s, std::vector<FPass>& passes, bool f32)
+// blocks: FP32 passes fold register-bit runs into PO_BLOCK ops
+int launch_passes(qsv_state* s, std::vector<FPass>& passes, bool f32, bool blocks)
 {
   if (passes.empty()) return QSV_OK;
   const int TB = s->prec == QSV_C128 ? kTileBitsC128 : kTileBitsC64;
@@ -721,7 +869,7 @@ int launch_passes(qsv_state* s, std::vector<FPass>& passes, bool f32)
     nb = TileNorm{s->norm_buf, s->norm_buf + ntiles, s->norm_buf + 2 * ntiles};
   }
   PProg* dev = nullptr;
-  int rc = upload_passes(s, passes, &dev);
+  int rc = upload_passes(s, passes, f32 && blocks, &dev);
   if (rc) return rc;
   for (size_t i = 0; i < passes.size(); ++i) {
     const PProg* p = dev + i;
@@ -1127,7 +1275,9 @@ std::vector<BGate> fold_gates(const std::vector<BGate>& in)
 
 // flags: QSV_FLAG_FP64_PASSES keeps complex64 pass arithmetic in FP64; by
 // default it is FP32, renormalised pass by pass (15 % faster per 30-qubit
-// layer: no conversions, same register budget). The gates are folded first.
+// layer: no conversions, same register budget), and its register-bit runs
+// become matrix-core blocks unless QSV_FLAG_VALU_PASSES. The gates are
+// folded first.
 int run_fused(qsv_state* s, const std::vector<BGate>& gs_in, uint32_t flags)
 {
   const std::vector<BGate> gs = fold_gates(gs_in);
@@ -1140,21 +1290,22 @@ int run_fused(qsv_state* s, const std::vector<BGate>& gs_in, uint32_t flags)
     return QSV_OK;
   }
   const bool f32 = s->prec == QSV_C64 && !(flags & QSV_FLAG_FP64_PASSES);
+  const bool blocks = !(flags & QSV_FLAG_VALU_PASSES);
   std::vector<FPass> passes;
   int rc = fuse_walk(
       s, gs, f32,
       [&](const FPass& p) {
         passes.push_back(p);
-        return passes.size() >= 512 ? launch_passes(s, passes, f32) : QSV_OK;
+        return passes.size() >= 512 ? launch_passes(s, passes, f32, blocks) : QSV_OK;
       },
       [&](size_t i) {
         end_run(passes);
-        const int r = launch_passes(s, passes, f32);
+        const int r = launch_passes(s, passes, f32, blocks);
         return r ? r : run_direct(s, gs[i]);
       });
   if (rc) return rc;
   end_run(passes);
-  return launch_passes(s, passes, f32);
+  return launch_passes(s, passes, f32, blocks);
 }
 
 // Pass shapes of grouped batch programs, by program structure: an optimiser
@@ -1212,7 +1363,7 @@ bool plan_group_passes(int n, int prec, const std::vector<BGate>& gs, std::vecto
   auto append = [&](const FPass& p) {
     // header and ops, then the layout table right after the last op; the
     // instance's first pass writes |0..0> itself
-    const size_t ops_end = compile_pass(p, prog);
+    const size_t ops_end = compile_pass(p, prog, false);
     if (offsets.empty()) prog.h.renorm |= FP_INIT_ZERO;
     const size_t lay_bytes = (size_t)prog.h.nlay * sizeof(PLayout);
     const size_t lay_at = (ops_end + 15) & ~(size_t)15;

@@ -172,6 +172,7 @@ enum {
   PO_BLOCKADE = 25,   // phase where exactly one of the bits aux (thread) / kc (slot) is set
   PO_RELAYOUT = 26,   // layouts tmask -> kc (PProg::lay), new register mask aux; only1: wave-local
   PO_SWAP = 27,       // exchange local bits kc and aux where the local bits tmask are set
+  PO_BLOCK = 28,      // dense 16x16 on the four register bits (FP32 passes only): table at byte offset aux
 };
 struct POp {
   int32_t code;
@@ -195,6 +196,10 @@ struct PLayout {
   uint64_t gs[16];
 };
 constexpr int kMaxPassLayouts = kMaxPassStages + 2;
+// PO_BLOCK tables per pass: 4 steps x 64 lanes of (re, im) FP32, lane l of
+// step kk holding U[l % 16][4 kk + l / 16] (the A operand of
+// v_mfma_f32_16x16x4f32), U indexed by slot.
+constexpr int kMaxPassBlocks = 8;
 struct PHead {
   int32_t nops;
   int32_t renorm;                   // FP_RENORM_* bits
@@ -212,6 +217,7 @@ struct PProg {
   PHead h;
   POp ops[kMaxPassOps];
   PLayout lay[kMaxPassLayouts];
+  float blk[kMaxPassBlocks][4 * 64 * 2];
 };
 // f32: complex64 arithmetic in FP32 (ignored for complex128, always FP64)
 void launch_pass(int prec, bool f32, void* psi, const PProg* dprog, uint64_t ntiles, const TileNorm& nb,

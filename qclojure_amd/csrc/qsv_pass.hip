@@ -419,6 +419,71 @@ __global__ __launch_bounds__(kThreads, sizeof(A) == 4 ? 5 : sizeof(R) == 4 ? 4 :
     local_pending = false;
   };
 
+  // PO_BLOCK (FP32 arithmetic, complex64 tiles): Y = U X for the wave's
+  // 16 x 64 matrix X (row = slot, column = lane) on the matrix cores, as four
+  // column blocks of 16 lanes, 16 v_mfma_f32_16x16x4f32 each (split real and
+  // imaginary parts). X crosses LDS at the current layout's own addresses of
+  // the wave's amplitudes, so the round trip is wave-local. The offsets are
+  // XOR-linear in the thread and slot bits: lane 16 cb + jc's slot s sits at
+  // (wave part ^ jc part ^ cb part) ^ ls[s], and ls[4 a + b] = ls[4 a] ^ ls[b].
+  auto block_op = [&](const POp& op) __attribute__((always_inline)) {
+    if constexpr (sizeof(A) == 4 && !SPLIT) {
+      typedef float f4 __attribute__((ext_vector_type(4)));
+      const uint32_t l = t & 63u, kr = l >> 4, jc = l & 15u;
+      const float2* tab = reinterpret_cast<const float2*>(reinterpret_cast<const char*>(prog) + op.aux);
+      float ur[4], ui[4], un[4];
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const float2 u = tab[kk * 64 + l];
+        ur[kk] = u.x;
+        ui[kk] = u.y;
+        un[kk] = -u.y;
+      }
+      const PLayout& L = lays[cur];
+      const uint32_t fb = lds_base(L);
+      each_slot([&](auto kk) __attribute__((always_inline)) {
+        constexpr int k = decltype(kk)::value;
+        st_c(fb ^ L.ls[k], xr[k], xi[k]);
+      });
+      lds_sync(true);
+      uint32_t pj = ((t >> 6) & 1u ? L.lt[6] : 0u) ^ ((t >> 7) & 1u ? L.lt[7] : 0u);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) pj ^= (jc >> j) & 1u ? L.lt[j] : 0u;
+      const uint32_t rin = (kr & 1u ? L.ls[1] : 0u) ^ (kr & 2u ? L.ls[2] : 0u);    // B rows 4 kk + kr
+      const uint32_t rout = (kr & 1u ? L.ls[4] : 0u) ^ (kr & 2u ? L.ls[8] : 0u);   // C rows 4 kr + e
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) {
+        const uint32_t pb = pj ^ (cb & 1 ? L.lt[4] : 0u) ^ (cb & 2 ? L.lt[5] : 0u);
+        float br[4], bi[4];
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+          const T v = ld_c(pb ^ rin ^ L.ls[4 * kk]);
+          br[kk] = v.x;
+          bi[kk] = v.y;
+        }
+        f4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+          cr = __builtin_amdgcn_mfma_f32_16x16x4f32(ur[kk], br[kk], cr, 0, 0, 0);
+          cr = __builtin_amdgcn_mfma_f32_16x16x4f32(un[kk], bi[kk], cr, 0, 0, 0);
+          ci = __builtin_amdgcn_mfma_f32_16x16x4f32(ur[kk], bi[kk], ci, 0, 0, 0);
+          ci = __builtin_amdgcn_mfma_f32_16x16x4f32(ui[kk], br[kk], ci, 0, 0, 0);
+        }
+        // element e of lane l is row 4 (l / 16) + e, column l % 16 of the block
+#pragma unroll
+        for (int e = 0; e < 4; ++e) st_c(pb ^ rout ^ L.ls[e], cr[e], ci[e]);
+      }
+      lds_sync(true);
+      each_slot([&](auto kk) __attribute__((always_inline)) {
+        constexpr int k = decltype(kk)::value;
+        const T v = ld_c(fb ^ L.ls[k]);
+        xr[k] = v.x;
+        xi[k] = v.y;
+      });
+      local_pending = true;
+    }
+  };
+
   const int rn = h.renorm;
   {
     uint64_t ga[kSlots];
@@ -517,6 +582,8 @@ __global__ __launch_bounds__(kThreads, sizeof(A) == 4 ? 5 : sizeof(R) == 4 ? 4 :
       relayout((int)op.tmask, (int)op.kc, op.only1 != 0);
       reg = op.aux;
       cur = (int)op.kc;
+    } else if (code == PO_BLOCK) {
+      block_op(op);
     } else {
       swap_op(op);
     }

@@ -159,6 +159,70 @@ def test_c64_fp32_passes_renormalise(eng):
             assert c["fidelity"] > 1 - 1e-6 and c["max_abs_diff"] < 1e-5, c
 
 
+def _block_program(rng, n, layers):
+    """Layers of U3 (as RZ RY RZ, folded into one dense general 2x2) on every
+    qubit, then a random matching of CZ, CNOT, CRX/CRY/CRZ, SWAP, Toffoli and
+    two-atom blockades: long runs of register-bit gates between gates with
+    thread-bit parts and layout barriers."""
+    rows = []
+    for _ in range(layers):
+        for q in range(n):
+            for k in ("RZ", "RY", "RZ"):
+                rows.append((OP[k], q, 0, 0, 0, float(rng.uniform(0, 2 * math.pi))))
+        perm = rng.permutation(n)
+        for j in range(0, n - 1, 2):
+            a, b = int(perm[j]), int(perm[j + 1])
+            r = rng.uniform()
+            ang = float(rng.uniform(0, 2 * math.pi))
+            if r < 0.35:
+                rows.append((OP["CZ"], a, b, 0, 0, 0.0))
+            elif r < 0.6:
+                rows.append((OP["CNOT"], a, b, 0, 0, 0.0))
+            elif r < 0.75:
+                rows.append((OP[rng.choice(["CRX", "CRY", "CRZ"])], a, b, 0, 0, ang))
+            elif r < 0.85:
+                rows.append((OP["SWAP"], a, b, 0, 0, 0.0))
+            elif r < 0.95:
+                c = int(rng.integers(n))
+                if c not in (a, b):
+                    rows.append((OP["TOFFOLI"], a, c, b, 0, 0.0))
+            else:
+                rows.append((OP["RYDBERG_BLOCKADE"], 0, 0, 0, (1 << a) | (1 << b), ang))
+    return qc.ops_from_tuples(rows)
+
+
+@pytest.mark.parametrize("n", [13, 17, 22])
+def test_c64_matrix_core_blocks(eng, oracle, n, monkeypatch):
+    """complex64 FP32 passes fold a stage's long register-bit runs into dense
+    16x16 blocks on the matrix cores (moving them past thread-bit gates on
+    other bits); QSV_PASS_BLOCKS=1 makes every run a block, and
+    QSV_FLAG_VALU_PASSES keeps every gate on the vector units. Each must reach
+    the oracle's state (complex128 fused passes at 22 qubits) at the complex64
+    bound, on random programs (controls on thread and register bits,
+    blockades, swaps, Toffolis), random layers and U3/CZ-style layers."""
+    from qclojure_amd._abi import QSV_FLAG_VALU_PASSES
+    rng = np.random.default_rng(1300 + n)
+    progs = (("random", _program(rng, n, 300)), ("layers", wl.random_layer_ops(n, 8, seed=n)),
+             ("u3", _block_program(rng, n, 12)))
+
+    def check(got, exp):
+        assert abs(np.vdot(exp, got)) ** 2 > 1 - 1e-6
+        assert np.max(np.abs(got - exp)) < 1e-5
+        assert abs(float(np.vdot(got, got).real) - 1.0) < 2e-6
+
+    for name, ops in progs:
+        note(f"c64 blocks n={n} {name} gates={len(ops)}")
+        exp = oracle.run(n, ops) if n <= 17 else _run(eng, n, ops, "c128", QSV_FLAG_FUSE)
+        check(_run(eng, n, ops, "c64", QSV_FLAG_FUSE | QSV_FLAG_VALU_PASSES), exp)
+        for cost in ("1", None):
+            if cost:
+                monkeypatch.setenv("QSV_PASS_BLOCKS", cost)
+            else:
+                monkeypatch.delenv("QSV_PASS_BLOCKS", raising=False)
+            check(_run(eng, n, ops, "c64", QSV_FLAG_FUSE), exp)
+        monkeypatch.delenv("QSV_PASS_BLOCKS", raising=False)
+
+
 def _fold_heavy_program(rng, n, layers):
     """Programs full of what the host folds and of what must stop a fold:
     RZ RY RZ chains, RZ on a CNOT control and RX on its target (both commute
