@@ -593,16 +593,17 @@ void make_layout(uint32_t reg, const int32_t pos[16], PLayout& L)
   }
 }
 
-// A PO_BLOCK costs a wave 64 v_mfma_f32_16x16x4f32 (128 flops per amplitude,
-// 0.9 ms per 2^30 amplitudes at the FP32 matrix peak) plus an LDS round trip,
-// whatever it absorbs, so it pays only for long runs of expensive ops. Vector
-// instructions per wave of each register-bit op in an FP32 pass (dense real,
-// dense general and two-sided diagonal measured with one op kind per pass;
-// the exchange and the one-sided diagonal estimated), and the least total a
-// block must replace (QSV_PASS_BLOCKS; 0 keeps every op on the vector units):
-// on complex64 30-qubit passes, blocks of three or more ops ran 15-40 %
-// slower than the vector units, blocks of six or more 2 % faster on U3/CZ
-// layers.
+// A PO_BLOCK costs a wave 48 v_mfma_f32_16x16x4f32 (three real 16 x 16 x 16
+// products per column block of 16 lanes: 0.66 ms per 2^30 amplitudes at the
+// FP32 matrix peak) plus an LDS round trip, whatever it absorbs, so it pays
+// only for long runs of expensive ops. Vector instructions per wave of each
+// register-bit op in an FP32 pass (dense real, dense general and two-sided
+// diagonal measured with one op kind per pass; the exchange and the one-sided
+// diagonal estimated), and the least total a block must replace
+// (QSV_PASS_BLOCKS; 0 keeps every op on the vector units). On complex64
+// 30-qubit passes a threshold of 650 ran U3/CZ layers 3.7 % and random
+// layers 0.8 % faster than the vector units alone; 450 broke even, and 300
+// (blocks of about three ops) ran 6-14 % slower.
 int reg_op_cost(const POp& op)
 {
   const int c = op.code;
@@ -617,7 +618,7 @@ int reg_op_cost(const POp& op)
 int pass_block_cost()
 {
   const char* e = std::getenv("QSV_PASS_BLOCKS");   // read per pass: a test may lower it
-  return e ? std::max(0, std::atoi(e)) : 850;
+  return e ? std::max(0, std::atoi(e)) : 650;
 }
 
 // Slot bits an op of a stage reads or writes; `reg_only`: it has no

@@ -421,8 +421,8 @@ __global__ __launch_bounds__(kThreads, sizeof(A) == 4 ? 5 : sizeof(R) == 4 ? 4 :
 
   // PO_BLOCK (FP32 arithmetic, complex64 tiles): Y = U X for the wave's
   // 16 x 64 matrix X (row = slot, column = lane) on the matrix cores, as four
-  // column blocks of 16 lanes, 16 v_mfma_f32_16x16x4f32 each (split real and
-  // imaginary parts). X crosses LDS at the current layout's own addresses of
+  // column blocks of 16 lanes, 12 v_mfma_f32_16x16x4f32 each (three real
+  // products per complex one). X crosses LDS at the current layout's own addresses of
   // the wave's amplitudes, so the round trip is wave-local. The offsets are
   // XOR-linear in the thread and slot bits: lane 16 cb + jc's slot s sits at
   // (wave part ^ jc part ^ cb part) ^ ls[s], and ls[4 a + b] = ls[4 a] ^ ls[b].
@@ -431,13 +431,13 @@ __global__ __launch_bounds__(kThreads, sizeof(A) == 4 ? 5 : sizeof(R) == 4 ? 4 :
       typedef float f4 __attribute__((ext_vector_type(4)));
       const uint32_t l = t & 63u, kr = l >> 4, jc = l & 15u;
       const float2* tab = reinterpret_cast<const float2*>(reinterpret_cast<const char*>(prog) + op.aux);
-      float ur[4], ui[4], un[4];
+      float ur[4], ui[4], us[4];
 #pragma unroll
       for (int kk = 0; kk < 4; ++kk) {
         const float2 u = tab[kk * 64 + l];
         ur[kk] = u.x;
         ui[kk] = u.y;
-        un[kk] = -u.y;
+        us[kk] = u.x + u.y;
       }
       const PLayout& L = lays[cur];
       const uint32_t fb = lds_base(L);
@@ -461,17 +461,18 @@ __global__ __launch_bounds__(kThreads, sizeof(A) == 4 ? 5 : sizeof(R) == 4 ? 4 :
           br[kk] = v.x;
           bi[kk] = v.y;
         }
-        f4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
+        // three real products (Gauss): P1 = Ur Xr, P2 = Ui Xi,
+        // P3 = (Ur + Ui)(Xr + Xi); Y = (P1 - P2) + i (P3 - P1 - P2)
+        f4 p1 = {0.f, 0.f, 0.f, 0.f}, p2 = {0.f, 0.f, 0.f, 0.f}, p3 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
-          cr = __builtin_amdgcn_mfma_f32_16x16x4f32(ur[kk], br[kk], cr, 0, 0, 0);
-          cr = __builtin_amdgcn_mfma_f32_16x16x4f32(un[kk], bi[kk], cr, 0, 0, 0);
-          ci = __builtin_amdgcn_mfma_f32_16x16x4f32(ur[kk], bi[kk], ci, 0, 0, 0);
-          ci = __builtin_amdgcn_mfma_f32_16x16x4f32(ui[kk], br[kk], ci, 0, 0, 0);
+          p1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ur[kk], br[kk], p1, 0, 0, 0);
+          p2 = __builtin_amdgcn_mfma_f32_16x16x4f32(ui[kk], bi[kk], p2, 0, 0, 0);
+          p3 = __builtin_amdgcn_mfma_f32_16x16x4f32(us[kk], br[kk] + bi[kk], p3, 0, 0, 0);
         }
         // element e of lane l is row 4 (l / 16) + e, column l % 16 of the block
 #pragma unroll
-        for (int e = 0; e < 4; ++e) st_c(pb ^ rout ^ L.ls[e], cr[e], ci[e]);
+        for (int e = 0; e < 4; ++e) st_c(pb ^ rout ^ L.ls[e], p1[e] - p2[e], p3[e] - p1[e] - p2[e]);
       }
       lds_sync(true);
       each_slot([&](auto kk) __attribute__((always_inline)) {
