@@ -306,11 +306,12 @@ int qsv_timing_read(qsv_state* s, qsv_kernel_stat out[QSV_KCLASS_COUNT], int res
 
 /* ---- fused-pass tuner ----------------------------------------------------
  * No counterpart in the reference (its simulator has no tunables). Times a
- * fused program of four random layers (a dense 2x2 on every qubit, then CNOTs
- * on a random matching) on a scratch state of `num_qubits` (16..30; 0 = 26) for every
+ * fused program of eight random layers (H, RX, RY or RZ on every qubit, then
+ * CNOTs on a random matching; device time of its kernels, host planning
+ * excluded) on a scratch state of `num_qubits` (16..30; 0 = 26) for every
  * candidate width of the contiguous index run a tile keeps (low_bits), and
- * makes the fastest the default for later fused programs of that precision
- * in this process. QSV_TILE_LOWBITS in the environment still overrides it. */
+ * makes the narrowest width within 1 % of the fastest the default for later
+ * fused programs of that precision in this process. QSV_TILE_LOWBITS in the environment still overrides it. */
 #define QSV_TUNE_MAX 8
 typedef struct qsv_tune_result {
   int32_t  low_bits;               /* the chosen width                      */

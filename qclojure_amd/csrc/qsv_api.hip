@@ -3313,54 +3313,56 @@ int qsv_tune_fused(int precision, int num_qubits, qsv_tune_result* out)
   qsv_state* s = nullptr;
   int rc = qsv_state_create(n, precision, &s);
   if (rc) return rc;
-  // the timed program: kTuneLayers layers of the dense unitary RZ(0.7) RY(0.3)
-  // on every bit, then CNOTs on a fixed pseudo-random matching per layer
-  constexpr int kTuneLayers = 4;
+  // the timed program: kTuneLayers layers of one gate per bit drawn from H,
+  // RX, RY and RZ (the random-layer workloads' mix), then CNOTs on a fixed
+  // pseudo-random matching per layer
+  constexpr int kTuneLayers = 8;
   std::vector<BGate> layer;
-  const double c = std::cos(0.15), sn = std::sin(0.15), zc = std::cos(0.35), zs = std::sin(0.35);
-  const double m[8] = {c * zc, -c * zs, -sn * zc, sn * zs, sn * zc, sn * zs, c * zc, c * zs};
   const double xm[8] = {0, 0, 1, 0, 1, 0, 0, 0};
+  const int kinds[4] = {QSV_OP_H, QSV_OP_RX, QSV_OP_RY, QSV_OP_RZ};
   std::vector<int> perm(n);
   uint64_t x = 0x2545F4914F6CDD1Dull;
+  auto next = [&x]() {
+    x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+    return x;
+  };
   for (int l = 0; l < kTuneLayers; ++l) {
-    for (int b = 0; b < n; ++b) perm[b] = b;
-    for (int b = n - 1; b > 0; --b) {
-      x ^= x << 13; x ^= x >> 7; x ^= x << 17;
-      std::swap(perm[b], perm[(int)(x % (uint64_t)(b + 1))]);
+    for (int b = 0; b < n; ++b) {
+      const uint64_t r = next();
+      double gm[8];
+      gate_matrix(kinds[r & 3], 0.1 + 6.0 * (double)((r >> 8) & 1023) / 1024.0, gm);
+      emit_ctl(layer, {}, b, gm);
     }
-    for (int b = 0; b < n; ++b) emit_ctl(layer, {}, b, m);
+    for (int b = 0; b < n; ++b) perm[b] = b;
+    for (int b = n - 1; b > 0; --b) std::swap(perm[b], perm[(int)(next() % (uint64_t)(b + 1))]);
     for (int k = 0; k + 1 < n; k += 2) emit_ctl(layer, {perm[k]}, perm[k + 1], xm);
   }
+  // each candidate's time: the device time of its kernels (per-launch events),
+  // not the host's fusion and compilation before them
   const int TB = precision == QSV_C128 ? kTileBitsC128 : kTileBitsC64;
   const int lo = low_bit_threshold(s), hi = std::min(TB - 4, lo + QSV_TUNE_MAX - 1);
-  hipEvent_t e0 = nullptr, e1 = nullptr;
   double best = 1e300;
   int bestL = 4;
-  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
-    rc = fail(QSV_E_DEVICE, "hipEventCreate failed");
-  } else {
-    for (int L = lo; L <= hi && !rc; ++L) {
-      s->tile_low = L;
-      double t = 1e300;
-      for (int rep = 0; rep < 3 && !rc; ++rep) {   // the first run is a warm-up
-        if (hipEventRecord(e0, s->stream) != hipSuccess) { rc = fail(QSV_E_DEVICE, "hipEventRecord failed"); break; }
-        if ((rc = run_fused(s, layer, 0))) break;
-        float ms = 0.f;
-        if (hipEventRecord(e1, s->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-          rc = fail(QSV_E_DEVICE, "tuning timing failed");
-          break;
-        }
-        if (rep) t = std::min(t, (double)ms / kTuneLayers);
-      }
-      if (rc) break;
-      out->candidate[out->n_candidates] = L;
-      out->ms[out->n_candidates++] = t;
-      if (t < best) { best = t; bestL = L; }
+  s->timing = true;
+  for (int L = lo; L <= hi && !rc; ++L) {
+    s->tile_low = L;
+    double t = 1e300;
+    for (int rep = 0; rep < 3 && !rc; ++rep) {   // the first run is a warm-up
+      std::memset(s->stats, 0, sizeof(s->stats));
+      if ((rc = run_fused(s, layer, 0)) || (rc = flush_timing(s))) break;
+      double ms = 0.0;
+      for (const auto& st : s->stats) ms += st.total_ms;
+      if (rep) t = std::min(t, ms / kTuneLayers);
     }
+    if (rc) break;
+    out->candidate[out->n_candidates] = L;
+    out->ms[out->n_candidates++] = t;
+    best = std::min(best, t);
   }
-  if (e0) hipEventDestroy(e0);
-  if (e1) hipEventDestroy(e1);
+  // the narrowest width within 1 % of the fastest: a narrower run leaves more
+  // tile bits for gates, which longer programs turn into fewer passes
+  for (int i = out->n_candidates - 1; i >= 0; --i)
+    if (out->ms[i] <= 1.01 * best) bestL = out->candidate[i];
   const int drc = qsv_state_destroy(s);
   if (rc) return rc;
   if (drc) return drc;

@@ -122,14 +122,16 @@ def test_small_states_fall_back(eng, oracle):
 
 @pytest.mark.parametrize("prec", ["c128", "c64"])
 def test_tuner_picks_a_candidate_and_results_hold(eng, prec):
-    """qsv_tune_fused times every candidate width and keeps the fastest; the
-    fused path under the tuned width still equals the per-gate path."""
+    """qsv_tune_fused times every candidate width and keeps the narrowest
+    within 1 % of the fastest; the fused path under the tuned width still
+    equals the per-gate path."""
     note(f"tune_fused prec={prec} n=20")
     res = eng.tune_fused(prec, 20)
     times = res["ms_per_layer"]
     assert len(times) >= 2 and res["low_bits"] in times
     assert all(t > 0 for t in times.values())
-    assert times[res["low_bits"]] == min(times.values())
+    assert times[res["low_bits"]] <= 1.01 * min(times.values())
+    assert all(times[w] > 1.01 * min(times.values()) for w in times if w < res["low_bits"])
     n = 21
     ops = wl.random_layer_ops(n, 6, seed=12)
     a = _run(eng, n, ops, prec)
