@@ -335,37 +335,56 @@ def bench_serving(engine, rates, n_req: int, n_plain: int, seed: int) -> dict:
 def bench_precision(engine, n: int, layers: int, seed: int) -> dict:
     """Seeded random product state, the reference QFT, then layers of random U3
     (as RZ RY RZ) and CZ on a random matching: complex64 vs complex128, both
-    through fused passes, compared every 10 layers."""
+    through fused passes, compared every 10 layers. A third state runs the
+    complex64 program with every gate on the vector units
+    (QSV_FLAG_VALU_PASSES), so the section also times the matrix-core blocks
+    of the FP32 passes against the vector units on the same chunks."""
     from qclojure_amd import algorithms as alg
     from qclojure_amd import circuit as qc
     from qclojure_amd import workloads as wl
-    from qclojure_amd._abi import QSV_FLAG_FUSE
+    from qclojure_amd._abi import QSV_FLAG_FUSE, QSV_FLAG_VALU_PASSES
     circ = wl.precision_circuit(n, layers, seed)
     ops = qc.decode_circuit(circ)
     prologue = 2 * n + len(alg.quantum_fourier_transform_circuit(n)["operations"])
     per_layer = 3 * n + n // 2
     a = engine.DeviceState(n, "c128")
     b = engine.DeviceState(n, "c64")
+    v = engine.DeviceState(n, "c64")
     rows = []
+    t_blocks = t_valu = 0.0
+
+    def timed_apply(s, part, flags):
+        s.sync()
+        t = time.perf_counter()
+        s.apply(part, flags)
+        s.sync()
+        return time.perf_counter() - t
+
     t0 = time.perf_counter()
     a.apply(ops[:prologue], QSV_FLAG_FUSE)
-    b.apply(ops[:prologue], QSV_FLAG_FUSE)
+    t_blocks += timed_apply(b, ops[:prologue], QSV_FLAG_FUSE)
+    t_valu += timed_apply(v, ops[:prologue], QSV_FLAG_FUSE | QSV_FLAG_VALU_PASSES)
     for L in range(0, layers, 10):
         chunk = ops[prologue + L * per_layer:prologue + min(layers, L + 10) * per_layer]
         a.apply(chunk, QSV_FLAG_FUSE)
-        b.apply(chunk, QSV_FLAG_FUSE)
+        t_blocks += timed_apply(b, chunk, QSV_FLAG_FUSE)
+        t_valu += timed_apply(v, chunk, QSV_FLAG_FUSE | QSV_FLAG_VALU_PASSES)
         c = b.compare(a)
+        cv = v.compare(a)
         rows.append({"layers": min(layers, L + 10), "fidelity": c["fidelity"],
                      "fidelity_normalized": c["fidelity"] / (c["norm2_self"] * c["norm2_other"]),
                      "max_abs_diff": c["max_abs_diff"], "norm_drift_c64": abs(c["norm2_self"] - 1.0),
-                     "norm_drift_c128": abs(c["norm2_other"] - 1.0)})
+                     "norm_drift_c128": abs(c["norm2_other"] - 1.0), "fidelity_valu_passes": cv["fidelity"]})
         log(f"bench: precision {rows[-1]}")
     dt = time.perf_counter() - t0
     a.destroy()
     b.destroy()
+    v.destroy()
     return {"qubits": n, "layers": layers, "gates": int(len(ops)), "seconds": round(dt, 2), "per_10_layers": rows,
             "min_fidelity": min(r["fidelity"] for r in rows),
-            "min_fidelity_normalized": min(r["fidelity_normalized"] for r in rows)}
+            "min_fidelity_normalized": min(r["fidelity_normalized"] for r in rows),
+            "c64_fused_s_matrix_core_blocks": round(t_blocks, 4), "c64_fused_s_vector_units_only": round(t_valu, 4),
+            "c64_blocks_speedup": round(t_valu / t_blocks, 4)}
 
 
 def bench_la(engine, n: int, seed: int) -> dict:
