@@ -362,11 +362,13 @@ def bench_precision(engine, n: int, layers: int, seed: int) -> dict:
 
     t0 = time.perf_counter()
     a.apply(ops[:prologue], QSV_FLAG_FUSE)
+    a.sync()   # the complex64 timings must not overlap the complex128 kernels
     t_blocks += timed_apply(b, ops[:prologue], QSV_FLAG_FUSE)
     t_valu += timed_apply(v, ops[:prologue], QSV_FLAG_FUSE | QSV_FLAG_VALU_PASSES)
     for L in range(0, layers, 10):
         chunk = ops[prologue + L * per_layer:prologue + min(layers, L + 10) * per_layer]
         a.apply(chunk, QSV_FLAG_FUSE)
+        a.sync()
         t_blocks += timed_apply(b, chunk, QSV_FLAG_FUSE)
         t_valu += timed_apply(v, chunk, QSV_FLAG_FUSE | QSV_FLAG_VALU_PASSES)
         c = b.compare(a)
