@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <initializer_list>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -228,11 +229,37 @@ inline bool is_one(double re, double im) { return re == 1.0 && im == 0.0; }
 // shorter than a 128 B line); bits at or above it are enumerated by insertion.
 inline int low_bit_threshold(const qsv_state* s) { return s->prec == QSV_C128 ? 3 : 4; }
 
+// Control bits of a gate, held inline: decoding and folding hundreds of
+// thousands of gates per batched call allocate nothing per gate. No gate
+// kind has more than two controls.
+struct CtlBits {
+  static constexpr int kMax = 4;
+  int n = 0;
+  int c[kMax] = {0, 0, 0, 0};
+  CtlBits() = default;
+  CtlBits(std::initializer_list<int> l)
+  {
+    for (int x : l)
+      if (n < kMax) c[n++] = x;
+  }
+  const int* begin() const { return c; }
+  const int* end() const { return c + n; }
+  size_t size() const { return (size_t)n; }
+  bool empty() const { return n == 0; }
+  bool operator==(const CtlBits& o) const
+  {
+    if (n != o.n) return false;
+    for (int i = 0; i < n; ++i)
+      if (c[i] != o.c[i]) return false;
+    return true;
+  }
+};
+
 // 2x2 `m` applied to target bit tb where all control bits are set (no controls
 // for a plain 1-qubit gate). Diagonal matrices take the phase kernel, which
 // touches only the |1> half of the target when m00 == 1.
 // Algorithmic bytes: each touched amplitude is read once and written once.
-int apply_ctl_bits(qsv_state* s, const std::vector<int>& cbits, int tb, const double m[8])
+int apply_ctl_bits(qsv_state* s, const CtlBits& cbits, int tb, const double m[8])
 {
   const uint64_t B = s->bytes_per_amp();
   const int thr = low_bit_threshold(s);
@@ -266,7 +293,7 @@ int apply_ctl_bits(qsv_state* s, const std::vector<int>& cbits, int tb, const do
 }
 
 // Exchange of index bits b1, b2 (times ph) where all control bits are set.
-int apply_swap_bits(qsv_state* s, const std::vector<int>& cbits, int b1, int b2, const double ph[2])
+int apply_swap_bits(qsv_state* s, const CtlBits& cbits, int b1, int b2, const double ph[2])
 {
   const uint64_t B = s->bytes_per_amp();
   const int thr = low_bit_threshold(s);
@@ -360,7 +387,7 @@ inline bool runs_alone(int kind) { return kind == BG_MEASURE || kind == FG_RENOR
 
 struct BGate {
   int kind = FG_DENSE;     // FG_DENSE, FG_PHASE, FG_SWAP, FG_BLOCKADE or BG_MEASURE
-  std::vector<int> cbits;  // control bits (all must be 1)
+  CtlBits cbits;           // control bits (all must be 1)
   int a = -1, b = -1;      // target bit / swap bits
   uint64_t bmask = 0;      // blockade index bits; measured qubit mask for BG_MEASURE
   bool phase_only = false;
@@ -378,10 +405,10 @@ uint64_t gate_bits(const BGate& g)
   return v;
 }
 
-void emit_ctl(std::vector<BGate>& out, std::vector<int> cbits, int tb, const double m[8])
+void emit_ctl(std::vector<BGate>& out, CtlBits cbits, int tb, const double m[8])
 {
   BGate g;
-  g.cbits = std::move(cbits);
+  g.cbits = cbits;
   g.a = tb;
   std::memcpy(g.m, m, sizeof(g.m));
   if (is_diag(m)) {
@@ -391,11 +418,11 @@ void emit_ctl(std::vector<BGate>& out, std::vector<int> cbits, int tb, const dou
   out.push_back(std::move(g));
 }
 
-void emit_swap(std::vector<BGate>& out, std::vector<int> cbits, int b1, int b2, double pr, double pi)
+void emit_swap(std::vector<BGate>& out, CtlBits cbits, int b1, int b2, double pr, double pi)
 {
   BGate g;
   g.kind = FG_SWAP;
-  g.cbits = std::move(cbits);
+  g.cbits = cbits;
   g.a = b1;
   g.b = b2;
   g.m[0] = pr;
@@ -1234,26 +1261,32 @@ std::vector<BGate> fold_gates(const std::vector<BGate>& in)
   }
   std::vector<BGate> out;
   out.reserve(a.size());
-  std::unordered_map<int, int> last;                // bit -> index in out of its pending one-qubit gate
-  std::unordered_map<int, std::vector<int>> since;  // bit -> gates touching it after that one
+  // per bit: index in out of its pending one-qubit gate (-1: none), and the
+  // gates touching the bit after that one (reused across calls)
+  int last[64];
+  std::fill(last, last + 64, -1);
+  thread_local std::vector<int> since[64];
+  for (auto& v : since) v.clear();
+  auto reset = [&]() {
+    std::fill(last, last + 64, -1);
+    for (auto& v : since) v.clear();
+  };
   for (BGate& g : a) {
     if (runs_alone(g.kind)) {
       out.push_back(std::move(g));
-      last.clear();
-      since.clear();
+      reset();
       continue;
     }
     if (is_1q(g)) {
       const int b = g.a;
-      auto it = last.find(b);
-      if (it != last.end()) {
+      if (last[b] >= 0) {
         bool ok = true;
         for (int j : since[b])
           if (!commutes_on_bit(out[j], b, g.m)) { ok = false; break; }
         if (ok) {
           double r[8];
-          mul2(g.m, out[it->second].m, r);
-          set_1q(out[it->second], r);
+          mul2(g.m, out[last[b]].m, r);
+          set_1q(out[last[b]], r);
           continue;
         }
       }
@@ -1268,7 +1301,7 @@ std::vector<BGate> fold_gates(const std::vector<BGate>& in)
     while (bits) {
       const int b = __builtin_ctzll(bits);
       bits &= bits - 1;
-      if (last.count(b)) since[b].push_back(idx);
+      if (last[b] >= 0) since[b].push_back(idx);
     }
   }
   return out;
@@ -2181,6 +2214,7 @@ int qsv_batch_expect_z(const qsv_batch_desc* d, double* energies)
   parallel([&](uint64_t i) {
     const int n = d->num_qubits[i];
     uint64_t mk = 0;
+    decoded[i].reserve(d->op_offsets[i + 1] - d->op_offsets[i] + 8);   // about one gate per op
     for (uint64_t k = d->op_offsets[i]; k < d->op_offsets[i + 1]; ++k) {
       if (d->ops[k].kind == QSV_OP_MEASURE) {
         dec_rc[i] = QSV_E_UNSUPPORTED;
