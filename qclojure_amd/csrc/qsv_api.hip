@@ -239,8 +239,11 @@ struct CtlBits {
   CtlBits() = default;
   CtlBits(std::initializer_list<int> l)
   {
-    for (int x : l)
-      if (n < kMax) c[n++] = x;
+    if (l.size() > (size_t)kMax) {   // an internal invariant: never silently drop a control
+      std::fprintf(stderr, "qsv: a gate with %zu controls (at most %d)\n", l.size(), kMax);
+      std::abort();
+    }
+    for (int x : l) c[n++] = x;
   }
   const int* begin() const { return c; }
   const int* end() const { return c + n; }
