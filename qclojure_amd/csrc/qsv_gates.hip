@@ -301,47 +301,43 @@ static inline void launch_chunks(uint64_t blocks, F&& launch)
     launch((unsigned)std::min<uint64_t>(kMaxGrid, blocks - c0), c0);
 }
 
+// Work per thread, from a 30-qubit in-place probe (GB/s of algorithmic
+// traffic): one pair per thread in XCD-contiguous workgroup order streams
+// complex64 pairs at 6437-6824 against 5780-6783 with two or four pairs per
+// thread (10-12 % more for partners 4 MiB or more apart) and complex128 pairs
+// within 1 % of the best shape at every distance; kernels that touch one
+// amplitude per element stream best with one element per thread for
+// complex128 (6764 against 6113 with four) and two for complex64 (6652
+// against 6499 with four).
+constexpr int kPairsPerThread = 1;
+template <typename R> constexpr int kEltsPerThread = sizeof(R) == 8 ? 1 : 2;
+
 void launch_gate1(int prec, void* psi, const GateArgs& g, const double m[8], hipStream_t st)
 {
   if (g.ins.count == 0) return;
   Mat2<double> M;
   for (int k = 0; k < 8; ++k) M.m[k] = m[k];
   if (g.lo_t == 0) {
-    // Pair partners 4 MiB or more apart stream fastest with four pairs in
-    // flight per thread in plain workgroup order; closer partners with two
-    // pairs and XCD-contiguous chunks (30-qubit probe, TB/s: >= 4 MiB 5.66-5.73
-    // vs 5.44-5.77; below, 5.83-5.97 vs 5.48-5.83).
-    const uint64_t dist = g.tbit * (prec == QSV_C128 ? 16u : 8u);
-    if (dist >= (4u << 20)) {
-      constexpr int PPT = 4;
-      launch_chunks(blocks_for(g.ins.count, PPT), [&](unsigned grid, uint64_t c0) {
-        if (prec == QSV_C128)
-          hipLaunchKernelGGL((k_gate1<double, PPT, false, false>), dim3(grid), dim3(kBlock), 0, st,
-                             (NV<double>::T*)psi, g, M, c0);
-        else
-          hipLaunchKernelGGL((k_gate1<float, PPT, false, false>), dim3(grid), dim3(kBlock), 0, st,
-                             (NV<float>::T*)psi, g, M, c0);
-      });
-    } else {
-      constexpr int PPT = 2;
-      launch_chunks(blocks_for(g.ins.count, PPT), [&](unsigned grid, uint64_t c0) {
-        if (prec == QSV_C128)
-          hipLaunchKernelGGL((k_gate1<double, PPT, false, true>), dim3(grid), dim3(kBlock), 0, st,
-                             (NV<double>::T*)psi, g, M, c0);
-        else
-          hipLaunchKernelGGL((k_gate1<float, PPT, false, true>), dim3(grid), dim3(kBlock), 0, st,
-                             (NV<float>::T*)psi, g, M, c0);
-      });
-    }
-  } else {
-    constexpr int PPT = 4;
+    constexpr int PPT = kPairsPerThread;
     launch_chunks(blocks_for(g.ins.count, PPT), [&](unsigned grid, uint64_t c0) {
       if (prec == QSV_C128)
-        hipLaunchKernelGGL((k_gate1<double, PPT, true, true>), dim3(grid), dim3(kBlock), 0, st, (NV<double>::T*)psi,
-                           g, M, c0);
+        hipLaunchKernelGGL((k_gate1<double, PPT, false, true>), dim3(grid), dim3(kBlock), 0, st,
+                           (NV<double>::T*)psi, g, M, c0);
       else
-        hipLaunchKernelGGL((k_gate1<float, PPT, true, true>), dim3(grid), dim3(kBlock), 0, st, (NV<float>::T*)psi,
-                           g, M, c0);
+        hipLaunchKernelGGL((k_gate1<float, PPT, false, true>), dim3(grid), dim3(kBlock), 0, st,
+                           (NV<float>::T*)psi, g, M, c0);
+    });
+  } else if (prec == QSV_C128) {
+    constexpr int EPT = kEltsPerThread<double>;
+    launch_chunks(blocks_for(g.ins.count, EPT), [&](unsigned grid, uint64_t c0) {
+      hipLaunchKernelGGL((k_gate1<double, EPT, true, true>), dim3(grid), dim3(kBlock), 0, st, (NV<double>::T*)psi,
+                         g, M, c0);
+    });
+  } else {
+    constexpr int EPT = kEltsPerThread<float>;
+    launch_chunks(blocks_for(g.ins.count, EPT), [&](unsigned grid, uint64_t c0) {
+      hipLaunchKernelGGL((k_gate1<float, EPT, true, true>), dim3(grid), dim3(kBlock), 0, st, (NV<float>::T*)psi,
+                         g, M, c0);
     });
   }
 }
@@ -349,14 +345,18 @@ void launch_gate1(int prec, void* psi, const GateArgs& g, const double m[8], hip
 void launch_phase(int prec, void* psi, const GateArgs& g, const double d0[2], const double d1[2], hipStream_t st)
 {
   if (g.ins.count == 0) return;
-  constexpr int EPT = 4;
   const Cplx<double> e0{d0[0], d0[1]}, e1{d1[0], d1[1]};
-  launch_chunks(blocks_for(g.ins.count, EPT), [&](unsigned grid, uint64_t c0) {
-    if (prec == QSV_C128)
+  if (prec == QSV_C128) {
+    constexpr int EPT = kEltsPerThread<double>;
+    launch_chunks(blocks_for(g.ins.count, EPT), [&](unsigned grid, uint64_t c0) {
       hipLaunchKernelGGL((k_phase<double, EPT>), dim3(grid), dim3(kBlock), 0, st, (NV<double>::T*)psi, g, e0, e1, c0);
-    else
+    });
+  } else {
+    constexpr int EPT = kEltsPerThread<float>;
+    launch_chunks(blocks_for(g.ins.count, EPT), [&](unsigned grid, uint64_t c0) {
       hipLaunchKernelGGL((k_phase<float, EPT>), dim3(grid), dim3(kBlock), 0, st, (NV<float>::T*)psi, g, e0, e1, c0);
-  });
+    });
+  }
 }
 
 void launch_swap2(int prec, void* psi, const SwapArgs& s, const double ph[2], hipStream_t st)
@@ -364,45 +364,39 @@ void launch_swap2(int prec, void* psi, const SwapArgs& s, const double ph[2], hi
   if (s.ins.count == 0) return;
   const Cplx<double> p{ph[0], ph[1]};
   const int mode = (s.hA && s.hB) ? 0 : (s.hA ? 1 : 2);
-  // the pair kernel's policy: partners at least 4 MiB apart stream best with
-  // four pairs per thread in plain order, closer ones with two pairs per
-  // thread in XCD-contiguous order
-  const uint64_t B = prec == QSV_C128 ? 16 : 8;
-  const uint64_t dist = mode == 0 ? (s.hA > s.hB ? s.hA - s.hB : s.hB - s.hA) : s.hA;
-  const bool far = mode != 2 && dist * B >= (4ull << 20);
-#define QSV_SWAP_LAUNCH(MODE, PPT, XCD)                                                                         \
+#define QSV_SWAP_LAUNCH(R, MODE, PPT)                                                                            \
   launch_chunks(blocks_for(s.ins.count, PPT), [&](unsigned grid, uint64_t c0) {                                 \
-    if (prec == QSV_C128)                                                                                       \
-      hipLaunchKernelGGL((k_swap2<double, PPT, MODE, XCD>), dim3(grid), dim3(kBlock), 0, st, (NV<double>::T*)psi, \
-                         s, p, c0);                                                                             \
-    else                                                                                                        \
-      hipLaunchKernelGGL((k_swap2<float, PPT, MODE, XCD>), dim3(grid), dim3(kBlock), 0, st, (NV<float>::T*)psi,   \
-                         s, p, c0);                                                                             \
+    hipLaunchKernelGGL((k_swap2<R, PPT, MODE, true>), dim3(grid), dim3(kBlock), 0, st, (NV<R>::T*)psi, s, p, c0); \
   })
-  if (mode == 0) {
-    if (far) QSV_SWAP_LAUNCH(0, 4, false);
-    else QSV_SWAP_LAUNCH(0, 2, true);
-  } else if (mode == 1) {
-    if (far) QSV_SWAP_LAUNCH(1, 4, false);
-    else QSV_SWAP_LAUNCH(1, 2, true);
+  // modes 0 and 1 move pairs; mode 2 (both bits lane bits) one amplitude per element
+  if (prec == QSV_C128) {
+    if (mode == 0) QSV_SWAP_LAUNCH(double, 0, kPairsPerThread);
+    else if (mode == 1) QSV_SWAP_LAUNCH(double, 1, kPairsPerThread);
+    else QSV_SWAP_LAUNCH(double, 2, kEltsPerThread<double>);
   } else {
-    QSV_SWAP_LAUNCH(2, 4, true);
+    if (mode == 0) QSV_SWAP_LAUNCH(float, 0, kPairsPerThread);
+    else if (mode == 1) QSV_SWAP_LAUNCH(float, 1, kPairsPerThread);
+    else QSV_SWAP_LAUNCH(float, 2, kEltsPerThread<float>);
   }
 #undef QSV_SWAP_LAUNCH
 }
 
 void launch_blockade(int prec, void* psi, uint64_t S, uint64_t bmask, const double ph[2], hipStream_t st)
 {
-  constexpr int EPT = 4;
   const Cplx<double> p{ph[0], ph[1]};
-  launch_chunks(blocks_for(S, EPT), [&](unsigned grid, uint64_t c0) {
-    if (prec == QSV_C128)
-      hipLaunchKernelGGL((k_blockade<double, EPT>), dim3(grid), dim3(kBlock), 0, st, (NV<double>::T*)psi, S, bmask, p,
-                         c0);
-    else
+  if (prec == QSV_C128) {
+    constexpr int EPT = kEltsPerThread<double>;
+    launch_chunks(blocks_for(S, EPT), [&](unsigned grid, uint64_t c0) {
+      hipLaunchKernelGGL((k_blockade<double, EPT>), dim3(grid), dim3(kBlock), 0, st, (NV<double>::T*)psi, S, bmask,
+                         p, c0);
+    });
+  } else {
+    constexpr int EPT = kEltsPerThread<float>;
+    launch_chunks(blocks_for(S, EPT), [&](unsigned grid, uint64_t c0) {
       hipLaunchKernelGGL((k_blockade<float, EPT>), dim3(grid), dim3(kBlock), 0, st, (NV<float>::T*)psi, S, bmask, p,
                          c0);
-  });
+    });
+  }
 }
 
 }  // namespace qsv
