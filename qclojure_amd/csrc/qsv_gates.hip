@@ -305,12 +305,12 @@ static inline void launch_chunks(uint64_t blocks, F&& launch)
 // traffic): one pair per thread in XCD-contiguous workgroup order streams
 // complex64 pairs at 6437-6824 against 5780-6783 with two or four pairs per
 // thread (10-12 % more for partners 4 MiB or more apart) and complex128 pairs
-// within 1 % of the best shape at every distance; kernels that touch one
-// amplitude per element stream best with one element per thread for
-// complex128 (6764 against 6113 with four) and two for complex64 (6652
-// against 6499 with four).
+// within 1 % of the best shape at every distance. Kernels that touch one
+// amplitude per element take one element per thread for complex128 (RZ on a
+// 30-qubit state 5.23 ms against 5.53 ms with four) and four for complex64
+// (2.66 ms against 2.90 ms with two).
 constexpr int kPairsPerThread = 1;
-template <typename R> constexpr int kEltsPerThread = sizeof(R) == 8 ? 1 : 2;
+template <typename R> constexpr int kEltsPerThread = sizeof(R) == 8 ? 1 : 4;
 
 void launch_gate1(int prec, void* psi, const GateArgs& g, const double m[8], hipStream_t st)
 {
